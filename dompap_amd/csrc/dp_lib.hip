@@ -1,0 +1,1205 @@
+// dp_lib.hip -- host side of libdompap_hip.so: context and device memory,
+// hiprtc compilation of the pair kernels, the per-step launch sequence (with
+// hipGraph capture of step blocks), and the C-ABI declared in
+// include/dompap_hip.h.
+#include "dp_kernels.h"
+
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "dp_embedded.inc"  // dp_src_device_h, dp_src_pair (generated at build time)
+#include "../../include/dompap_hip.h"
+
+#define DP_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+std::string g_create_error;
+
+int rs_for_dim(int D) {
+  int rs = 2;
+  while (rs < D + 1) rs *= 2;
+  return rs;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+// Exact thresholds (monotone predicates over non-negative doubles).
+double bits2d(uint64_t b) { double d; std::memcpy(&d, &b, 8); return d; }
+uint64_t d2bits(double d) { uint64_t b; std::memcpy(&b, &d, 8); return b; }
+
+// Smallest s with !(sqrt(s) < cutoff): then  s < t  <=>  sqrt(s) < cutoff.
+double nl_threshold(double cutoff) {
+  auto inside = [&](double s) { return std::sqrt(s) < cutoff; };
+  if (!inside(0.0)) return 0.0;
+  const double inf = INFINITY;
+  if (inside(inf)) return inf;
+  uint64_t lo = 0, hi = d2bits(inf);
+  while (hi - lo > 1) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (inside(bits2d(mid))) lo = mid; else hi = mid;
+  }
+  return bits2d(hi);
+}
+
+// Largest s with !(sqrt(s)/sigma > r_cut): then  s > t  <=>  sqrt(s)/sigma > r_cut.
+double cut_threshold(double sigma, double r_cut) {
+  auto outside = [&](double s) { return std::sqrt(s) / sigma > r_cut; };
+  if (outside(0.0)) return -1.0;
+  const double inf = INFINITY;
+  if (!outside(inf)) return inf;
+  uint64_t lo = 0, hi = d2bits(inf);
+  while (hi - lo > 1) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (outside(bits2d(mid))) hi = mid; else lo = mid;
+  }
+  return bits2d(lo);
+}
+
+struct CodeCache {
+  std::mutex mu;
+  std::map<std::string, std::vector<char>> code;
+};
+CodeCache& code_cache() {
+  static CodeCache c;
+  return c;
+}
+
+
+std::string pair_source(int D, int mode, int ntypes, int even, int rs, int accumulate, const char* device_src) {
+  std::string src;
+  char buf[256];
+  snprintf(buf, sizeof(buf),
+           "#define DP_D %d\n#define DP_MODE %d\n#define DP_NTYPES %d\n#define DP_EVEN %d\n#define DP_RS %d\n"
+           "#define DP_ACC %d\n",
+           D, mode, ntypes, even ? 1 : 0, rs, (mode == DP_MIXED) ? accumulate : 0);
+  src += buf;
+  src += "#include \"dp_device.h\"\n";
+  src += device_src;
+  src += "\n";
+  src += dp_src_pair;
+  return src;
+}
+
+// hiprtc compile for gfx950 (no device needed), memoised per process.
+int compile_cached(const std::string& src, std::vector<char>& code, std::string& log) {
+  {
+    std::lock_guard<std::mutex> lk(code_cache().mu);
+    auto it = code_cache().code.find(src);
+    if (it != code_cache().code.end()) {
+      code = it->second;
+      return DP_OK;
+    }
+  }
+  hiprtcProgram prog;
+  const char* hdrs[] = {dp_src_device_h};
+  const char* names[] = {"dp_device.h"};
+  if (hiprtcCreateProgram(&prog, src.c_str(), "dp_pair.hip", 1, hdrs, names) != HIPRTC_SUCCESS) {
+    log = "hiprtcCreateProgram failed";
+    return DP_ERR_RTC;
+  }
+  // packed-FP32 instructions are disabled for all HIP code of this project
+  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=fast",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  hiprtcResult rr = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+  size_t ls = 0;
+  hiprtcGetProgramLogSize(prog, &ls);
+  log.assign(ls, '\0');
+  if (ls) hiprtcGetProgramLog(prog, &log[0]);
+  if (rr != HIPRTC_SUCCESS) {
+    hiprtcDestroyProgram(&prog);
+    return DP_ERR_RTC;
+  }
+  size_t cs = 0;
+  hiprtcGetCodeSize(prog, &cs);
+  code.resize(cs);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  std::lock_guard<std::mutex> lk(code_cache().mu);
+  code_cache().code[src] = code;
+  return DP_OK;
+}
+
+}  // namespace
+
+struct dp_ctx {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  int D = 3, mode = DP_FP64, R = 1, rs = 4;
+  i64 N = 0, Ns = 0;
+  int K = 0, order = DP_ORDER_X_FASTEST;
+  i64 Ctot = 0, ntiles = 0;
+  std::string err;
+
+  // host mirrors of parameters
+  std::vector<i64> offsets;
+  std::vector<double> box, cutoff, skin, dt, T, tau;
+  bool have_box = false, have_nl_params = false, have_integ = false;
+  std::vector<DpRep> reps_h;
+  DpPairTab tab_h;
+  int ntypes = 1;
+  bool have_pot = false;
+  double pot_rcut = 1.0;
+  bool reps_dirty = true, tab_dirty = true;
+
+  // device memory
+  DpRep* reps_d = nullptr;
+  DpPairTab* tab_d = nullptr;
+  DpCtl* ctl_d = nullptr;
+  DpCtl* ctl_h = nullptr;  // pinned
+  u64* dispmax_d = nullptr;
+  i64* nlupd_d = nullptr;
+  double* x[2] = {nullptr, nullptr};
+  float* xs[2] = {nullptr, nullptr};
+  double *v = nullptr, *beta = nullptr, *mass = nullptr, *xref = nullptr, *F = nullptr;
+  double *E = nullptr, *W = nullptr, *LAP = nullptr, *xi = nullptr, *red = nullptr;
+  i32 *img = nullptr, *type = nullptr, *orig = nullptr, *rep = nullptr, *cellpack = nullptr;
+  i32 *cell_of = nullptr, *rank = nullptr, *tmp = nullptr, *perm = nullptr, *nlcnt = nullptr;
+  i32 *ccount = nullptr, *cstart = nullptr, *tiles = nullptr;
+  u32* nl = nullptr;
+  char* scratch = nullptr;
+  char* stage = nullptr;
+  int cur = 0;
+
+  // pair kernels
+  hipModule_t mod = nullptr;
+  hipFunction_t f_force = nullptr, f_obs = nullptr, f_gjf = nullptr, f_lf = nullptr;
+  hipFunction_t f_int_gjf = nullptr, f_int_lf = nullptr;
+
+  // state flags
+  bool nl_built = false;
+  bool pos_dirty = true;
+  u64 seed = 0x5eed0d0bull;
+
+  // graphs
+  int graph_block = 32;
+  hipGraphExec_t gexec = nullptr;
+  int gexec_integ = 0;
+  bool graph_dirty = true;
+
+  // kernel timing
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_free;
+  double time_ms = 0.0;
+  i64 time_launches = 0;
+
+  int fail(int code, const char* fmt, ...) {
+    char buf[2048];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+  }
+};
+
+#define HIPCHK(call)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) return ctx->fail(DP_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHK(expr)               \
+  do {                          \
+    int rc_ = (expr);           \
+    if (rc_ != DP_OK) return rc_; \
+  } while (0)
+
+namespace {
+
+int grid_for(i64 n, int cap = 2048) {
+  i64 b = (n + DP_BLOCK - 1) / DP_BLOCK;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (int)b;
+}
+
+template <typename T>
+int dalloc(dp_ctx* ctx, T*& p, size_t count) {
+  dfree(p);
+  if (count == 0) count = 1;
+  HIPCHK(hipMalloc((void**)&p, count * sizeof(T)));
+  return DP_OK;
+}
+
+int release_graph(dp_ctx* ctx) {
+  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  ctx->gexec = nullptr;
+  ctx->graph_dirty = true;
+  return DP_OK;
+}
+
+// Cells and replica table from box / cutoff.
+int refresh_reps(dp_ctx* ctx) {
+  if (!ctx->reps_dirty) return DP_OK;
+  if (!ctx->have_box) return ctx->fail(DP_ERR_STATE, "dp_set_replicas has not been called");
+  if (!ctx->have_nl_params) return ctx->fail(DP_ERR_STATE, "dp_set_neighbor_params has not been called");
+  const int D = ctx->D;
+  const int bits = (D == 1) ? 31 : 32 / D;
+  const i64 maxc = (D == 1) ? (i64)0x7fffffff : ((i64)1 << bits) - 1;
+  ctx->reps_h.assign(ctx->R, DpRep{});
+  i64 c0 = 0;
+  for (int r = 0; r < ctx->R; ++r) {
+    DpRep& P = ctx->reps_h[r];
+    std::memset(&P, 0, sizeof(P));
+    P.p0 = ctx->offsets[r];
+    P.n = ctx->offsets[r + 1] - ctx->offsets[r];
+    const double cut = ctx->cutoff[r];
+    i64 nc[DP_DMAX];
+    i64 ncell = 1;
+    for (int d = 0; d < D; ++d) {
+      const double L = ctx->box[(size_t)r * D + d];
+      if (!(L > 0.0) || !std::isfinite(L)) return ctx->fail(DP_ERR_ARG, "box length must be positive and finite");
+      double q = (cut > 0.0 && std::isfinite(cut)) ? std::floor(L / cut) : 1.0;
+      if (!(q >= 3.0)) q = 1.0;
+      if (q > (double)maxc) q = (double)maxc;
+      nc[d] = (i64)q;
+      ncell *= nc[d];
+    }
+    // keep the grid to at most ~8 cells per particle (sparse systems)
+    const i64 cap = std::max<i64>(64, 8 * std::max<i64>(P.n, 1));
+    while (ncell > cap) {
+      ncell = 1;
+      for (int d = 0; d < D; ++d) {
+        if (nc[d] > 1) nc[d] = std::max<i64>(3, nc[d] / 2);
+        if (nc[d] == 3 && ncell * 3 > cap) nc[d] = 1;
+        ncell *= nc[d];
+      }
+      bool all_small = true;
+      for (int d = 0; d < D; ++d) all_small &= (nc[d] <= 3);
+      if (all_small) break;
+    }
+    ncell = 1;
+    for (int d = 0; d < D; ++d) ncell *= nc[d];
+    P.c0 = c0;
+    P.ncell = ncell;
+    c0 += ncell;
+    P.collapsed_mask = 0;
+    for (int d = 0; d < D; ++d) {
+      const double L = ctx->box[(size_t)r * D + d];
+      P.nc[d] = (i32)nc[d];
+      P.L[d] = L;
+      P.halfL[d] = L / 2;
+      P.w[d] = L / (double)nc[d];
+      P.inv_w[d] = (double)nc[d] / L;
+      const float hi = (float)P.w[d];
+      P.wf_hi[d] = hi;
+      P.wf_lo[d] = (float)(P.w[d] - (double)hi);
+      P.Lf[d] = (float)L;
+      P.invLf[d] = (float)(1.0 / L);
+      if (nc[d] == 1) P.collapsed_mask |= (1 << d);
+    }
+    P.nl_thresh = nl_threshold(cut);
+    P.skin_half_sq = (0.5 * ctx->skin[r]) * (0.5 * ctx->skin[r]);
+    P.dt = ctx->have_integ ? ctx->dt[r] : 0.01;
+    P.T = ctx->have_integ ? ctx->T[r] : 1.0;
+    P.tau = ctx->have_integ ? ctx->tau[r] : 0.1;
+  }
+  if (c0 >= (i64)0x7fffffff) return ctx->fail(DP_ERR_ARG, "too many cells (%lld)", (long long)c0);
+  if (c0 != ctx->Ctot) {
+    ctx->Ctot = c0;
+    ctx->ntiles = (c0 + DP_SCAN_TILE - 1) / DP_SCAN_TILE;
+    CHK(dalloc(ctx, ctx->ccount, (size_t)c0));
+    CHK(dalloc(ctx, ctx->cstart, (size_t)c0 + 1));
+    CHK(dalloc(ctx, ctx->tiles, (size_t)ctx->ntiles + 1));
+    release_graph(ctx);
+  }
+  HIPCHK(hipMemcpyAsync(ctx->reps_d, ctx->reps_h.data(), sizeof(DpRep) * ctx->R, hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));  // reps_h may change on the next call
+  ctx->reps_dirty = false;
+  ctx->tab_dirty = true;  // guard bands depend on cell widths
+  return DP_OK;
+}
+
+// Guard band for the FP32 cut test: bound on |s32 - s| for s near t.
+int refresh_tab(dp_ctx* ctx) {
+  if (!ctx->tab_dirty) return DP_OK;
+  if (!ctx->have_pot) return ctx->fail(DP_ERR_STATE, "no potential loaded (dp_load_potential)");
+  double xr_max = 0.0;
+  for (int r = 0; r < ctx->R; ++r) {
+    for (int d = 0; d < ctx->D; ++d) {
+      const DpRep& P = ctx->reps_h[r];
+      const double b = (P.nc[d] == 1) ? P.L[d] : 0.5 * P.w[d] + 0.5 * ctx->skin[r];
+      xr_max = std::max(xr_max, b);
+    }
+  }
+  DpPairTab& T = ctx->tab_h;
+  const int nt = ctx->ntypes;
+  for (int a = 0; a < nt; ++a)
+    for (int b = 0; b < nt; ++b) {
+      const int k = a * nt + b;
+      const double t = T.cut_thresh[k];
+      T.thresh_f[k] = (float)t;
+      if (!std::isfinite(t)) {
+        T.guard_lo_f[k] = INFINITY;
+        T.guard_hi_f[k] = INFINITY;
+        continue;
+      }
+      if (t < 0) {
+        T.guard_lo_f[k] = -1.0f;
+        T.guard_hi_f[k] = -1.0f;
+        continue;
+      }
+      const double e32 = std::ldexp(1.0, -24);
+      const double st = std::sqrt(t);
+      const double dd = 4.0 * e32 * (xr_max + st);
+      double g = 2.0 * st * std::sqrt((double)ctx->D) * dd + 8.0 * e32 * t;
+      g *= 4.0;  // safety factor
+      float lo = (float)(t - g), hi = (float)(t + g);
+      lo = std::nextafter(lo, -INFINITY);
+      hi = std::nextafter(hi, INFINITY);
+      if (lo < 0) lo = 0;
+      T.guard_lo_f[k] = lo;
+      T.guard_hi_f[k] = hi;
+    }
+  HIPCHK(hipMemcpyAsync(ctx->tab_d, &ctx->tab_h, sizeof(DpPairTab), hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->tab_dirty = false;
+  return DP_OK;
+}
+
+int prepare(dp_ctx* ctx) {
+  CHK(refresh_reps(ctx));
+  CHK(refresh_tab(ctx));
+  if (!ctx->mod) return ctx->fail(DP_ERR_STATE, "no potential loaded (dp_load_potential)");
+  return DP_OK;
+}
+
+DpPairArgs pair_args(dp_ctx* ctx, int cur) {
+  DpPairArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.x = ctx->x[cur];
+  a.xs = ctx->xs[cur];
+  a.nl = ctx->nl;
+  a.nlcnt = ctx->nlcnt;
+  a.type = ctx->type;
+  a.rep = ctx->rep;
+  a.reps = ctx->reps_d;
+  a.tab = ctx->tab_d;
+  a.N = ctx->N;
+  a.Ns = ctx->Ns;
+  a.x_out = ctx->x[1 - cur];
+  a.xs_out = ctx->xs[1 - cur];
+  a.v = ctx->v;
+  a.beta = ctx->beta;
+  a.img = ctx->img;
+  a.mass = ctx->mass;
+  a.xref = ctx->xref;
+  a.cellpack = ctx->cellpack;
+  a.orig = ctx->orig;
+  a.xi = nullptr;
+  a.seed = ctx->seed;
+  a.ctl = ctx->ctl_d;
+  a.dispmax = ctx->dispmax_d;
+  return a;
+}
+
+int launch_pair(dp_ctx* ctx, hipFunction_t f, DpPairArgs& a) {
+  const unsigned grid = (unsigned)((ctx->N + DP_BLOCK - 1) / DP_BLOCK);
+  void* params[] = {&a};
+  HIPCHK(hipModuleLaunchKernel(f, grid, 1, 1, DP_BLOCK, 1, 1, 0, ctx->st, params, nullptr));
+  return DP_OK;
+}
+
+template <int D>
+int launch_rebuild_t(dp_ctx* ctx) {
+  const int cur = ctx->cur;
+  const i64 N = ctx->N;
+  const int gN = grid_for(N);
+  hipStream_t st = ctx->st;
+  k_fill_i32<<<grid_for(ctx->Ctot), DP_BLOCK, 0, st>>>(ctx->ccount, ctx->Ctot, 0, ctx->ctl_d);
+  k_bin<D><<<gN, DP_BLOCK, 0, st>>>(ctx->x[cur], ctx->rep, ctx->reps_d, N, ctx->cell_of, ctx->rank, ctx->ccount,
+                                     ctx->ctl_d);
+  k_scan_reduce<<<grid_for(ctx->ntiles * DP_BLOCK), DP_BLOCK, 0, st>>>(ctx->ccount, ctx->Ctot, ctx->tiles, ctx->ctl_d);
+  k_scan_top<<<1, DP_BLOCK, 0, st>>>(ctx->tiles, ctx->ntiles, ctx->ctl_d);
+  k_scan_down<<<grid_for(ctx->ntiles * DP_BLOCK), DP_BLOCK, 0, st>>>(ctx->ccount, ctx->Ctot, ctx->tiles, ctx->cstart,
+                                                                    N, ctx->ctl_d);
+  k_scatter<<<gN, DP_BLOCK, 0, st>>>(ctx->cell_of, ctx->rank, ctx->cstart, N, ctx->tmp, ctx->ctl_d);
+  k_cellsort<<<gN, DP_BLOCK, 0, st>>>(ctx->tmp, ctx->cell_of, ctx->cstart, ctx->orig, N, ctx->perm, ctx->ctl_d);
+  k_permute<D><<<gN, DP_BLOCK, 0, st>>>(ctx->perm, N, ctx->x[cur], ctx->v, ctx->beta, ctx->mass, ctx->img,
+                                         ctx->type, ctx->orig, ctx->cell_of, ctx->scratch, ctx->ctl_d);
+  k_unpermute_store<D><<<gN, DP_BLOCK, 0, st>>>(N, ctx->x[cur], ctx->v, ctx->beta, ctx->mass, ctx->img, ctx->type,
+                                                 ctx->orig, ctx->cell_of, ctx->scratch, ctx->xref, ctx->cellpack,
+                                                 ctx->xs[cur], ctx->rs, ctx->rep, ctx->reps_d, ctx->ctl_d);
+  k_nl_emit<D><<<grid_for(N, 8192), DP_BLOCK, 0, st>>>(ctx->x[cur], ctx->cellpack, ctx->cstart, ctx->rep,
+                                                       ctx->reps_d, N, ctx->Ns, ctx->K, ctx->order, ctx->nl,
+                                                       ctx->nlcnt, ctx->ctl_d);
+  HIPCHK(hipGetLastError());
+  return DP_OK;
+}
+
+int launch_rebuild(dp_ctx* ctx) {
+  switch (ctx->D) {
+    case 1: return launch_rebuild_t<1>(ctx);
+    case 2: return launch_rebuild_t<2>(ctx);
+    case 3: return launch_rebuild_t<3>(ctx);
+    case 4: return launch_rebuild_t<4>(ctx);
+    case 5: return launch_rebuild_t<5>(ctx);
+    case 6: return launch_rebuild_t<6>(ctx);
+    case 7: return launch_rebuild_t<7>(ctx);
+    case 8: return launch_rebuild_t<8>(ctx);
+  }
+  return ctx->fail(DP_ERR_ARG, "dimension %d unsupported", ctx->D);
+}
+
+template <int D>
+int launch_skin_check_t(dp_ctx* ctx) {
+  k_fill_u64<<<1, DP_BLOCK, 0, ctx->st>>>(ctx->dispmax_d, ctx->R, 0ull);
+  k_skin_check<D><<<grid_for(ctx->N), DP_BLOCK, 0, ctx->st>>>(ctx->x[ctx->cur], ctx->xref, ctx->rep, ctx->reps_d,
+                                                              ctx->N, ctx->dispmax_d);
+  HIPCHK(hipGetLastError());
+  return DP_OK;
+}
+
+int launch_skin_check(dp_ctx* ctx) {
+  switch (ctx->D) {
+    case 1: return launch_skin_check_t<1>(ctx);
+    case 2: return launch_skin_check_t<2>(ctx);
+    case 3: return launch_skin_check_t<3>(ctx);
+    case 4: return launch_skin_check_t<4>(ctx);
+    case 5: return launch_skin_check_t<5>(ctx);
+    case 6: return launch_skin_check_t<6>(ctx);
+    case 7: return launch_skin_check_t<7>(ctx);
+    case 8: return launch_skin_check_t<8>(ctx);
+  }
+  return ctx->fail(DP_ERR_ARG, "dimension %d unsupported", ctx->D);
+}
+
+template <int D>
+int launch_records_t(dp_ctx* ctx) {
+  if (ctx->mode != DP_MIXED || !ctx->nl_built) return DP_OK;
+  k_make_records<D><<<grid_for(ctx->N), DP_BLOCK, 0, ctx->st>>>(ctx->N, ctx->x[ctx->cur], ctx->type, ctx->cellpack,
+                                                                ctx->xs[ctx->cur], ctx->rs, ctx->rep, ctx->reps_d);
+  HIPCHK(hipGetLastError());
+  return DP_OK;
+}
+
+int launch_records(dp_ctx* ctx) {
+  switch (ctx->D) {
+    case 1: return launch_records_t<1>(ctx);
+    case 2: return launch_records_t<2>(ctx);
+    case 3: return launch_records_t<3>(ctx);
+    case 4: return launch_records_t<4>(ctx);
+    case 5: return launch_records_t<5>(ctx);
+    case 6: return launch_records_t<6>(ctx);
+    case 7: return launch_records_t<7>(ctx);
+    case 8: return launch_records_t<8>(ctx);
+  }
+  return ctx->fail(DP_ERR_ARG, "dimension %d unsupported", ctx->D);
+}
+
+int launch_prologue(dp_ctx* ctx, int advance) {
+  k_prologue<<<1, 64, 0, ctx->st>>>(ctx->ctl_d, ctx->dispmax_d, ctx->reps_d, ctx->R, ctx->nlupd_d, advance);
+  HIPCHK(hipGetLastError());
+  return DP_OK;
+}
+
+// Read the control block; report overflow / non-finite positions.
+int check_ctl(dp_ctx* ctx) {
+  HIPCHK(hipMemcpyAsync(ctx->ctl_h, ctx->ctl_d, sizeof(DpCtl), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  if (ctx->ctl_h->overflow || ctx->ctl_h->nonfinite) {
+    const bool ov = ctx->ctl_h->overflow != 0;
+    const int mc = ctx->ctl_h->max_count;
+    // clear the sticky flags
+    DpCtl c = *ctx->ctl_h;
+    c.overflow = 0;
+    c.nonfinite = 0;
+    HIPCHK(hipMemcpy(ctx->ctl_d, &c, sizeof(DpCtl), hipMemcpyHostToDevice));
+    if (ov)
+      return ctx->fail(DP_ERR_OVERFLOW,
+                       "Number of neighbors (%d) exceeds max_number_of_neighbors=%d; "
+                       "increase max_number_of_neighbors",
+                       mc, ctx->K);
+    return ctx->fail(DP_ERR_NONFINITE, "non-finite particle position encountered");
+  }
+  return DP_OK;
+}
+
+// Skin check from the current positions when the host changed them, and a
+// forced rebuild if no list exists yet.
+int pre_step_checks(dp_ctx* ctx) {
+  if (!ctx->nl_built) {
+    k_request_rebuild<<<1, 1, 0, ctx->st>>>(ctx->ctl_d);
+    ctx->nl_built = true;  // the first prologue below builds it
+    ctx->pos_dirty = false;
+    return DP_OK;
+  }
+  if (ctx->pos_dirty) {
+    CHK(launch_skin_check(ctx));
+    ctx->pos_dirty = false;
+  }
+  return DP_OK;
+}
+
+int launch_one_step(dp_ctx* ctx, int integ, const double* xi_dev) {
+  CHK(launch_prologue(ctx, 1));
+  CHK(launch_rebuild(ctx));
+  DpPairArgs a = pair_args(ctx, ctx->cur);
+  a.xi = xi_dev;
+  hipFunction_t f = (integ == DP_INTEGRATOR_GJF) ? ctx->f_gjf : ctx->f_lf;
+  if (ctx->timing) {
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    if (!ctx->ev_free.empty()) {
+      ev = ctx->ev_free.back();
+      ctx->ev_free.pop_back();
+    } else {
+      HIPCHK(hipEventCreate(&ev.first));
+      HIPCHK(hipEventCreate(&ev.second));
+    }
+    HIPCHK(hipEventRecord(ev.first, ctx->st));
+    CHK(launch_pair(ctx, f, a));
+    HIPCHK(hipEventRecord(ev.second, ctx->st));
+    ctx->ev_pending.push_back(ev);
+  } else {
+    CHK(launch_pair(ctx, f, a));
+  }
+  ctx->cur = 1 - ctx->cur;
+  return DP_OK;
+}
+
+int resolve_timing(dp_ctx* ctx) {
+  if (ctx->ev_pending.empty()) return DP_OK;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  for (auto& ev : ctx->ev_pending) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ev.first, ev.second));
+    ctx->time_ms += ms;
+    ctx->time_launches += 1;
+    ctx->ev_free.push_back(ev);
+  }
+  ctx->ev_pending.clear();
+  return DP_OK;
+}
+
+int capture_graph(dp_ctx* ctx, int integ) {
+  release_graph(ctx);
+  hipGraph_t g = nullptr;
+  const int saved_cur = ctx->cur;
+  HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+  int rc = DP_OK;
+  for (int s = 0; s < ctx->graph_block && rc == DP_OK; ++s) rc = launch_one_step(ctx, integ, nullptr);
+  hipError_t e = hipStreamEndCapture(ctx->st, &g);
+  ctx->cur = saved_cur;
+  if (rc != DP_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return ctx->fail(DP_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    ctx->gexec = nullptr;
+    return ctx->fail(DP_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  }
+  ctx->gexec_integ = integ;
+  ctx->graph_dirty = false;
+  return DP_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+DP_API int dp_version(void) { return 100; }
+
+
+DP_API int dp_compile_check(int dim, int precision, int ntypes, int even, const char* device_src, int accumulate,
+                            char* log, int64_t log_len, int64_t* code_size) {
+  if (dim < 1 || dim > DP_DMAX || !device_src) return DP_ERR_ARG;
+  if (ntypes < 1 || ntypes > DP_MAXTYPES) return DP_ERR_ARG;
+  std::string src = pair_source(dim, precision, ntypes, even, rs_for_dim(dim), accumulate, device_src);
+  std::vector<char> code;
+  std::string lg;
+  int rc = compile_cached(src, code, lg);
+  if (log && log_len > 0) {
+    std::strncpy(log, lg.c_str(), (size_t)log_len - 1);
+    log[log_len - 1] = '\0';
+  }
+  if (code_size) *code_size = (int64_t)code.size();
+  return rc;
+}
+
+DP_API int dp_compile_code(int dim, int precision, int ntypes, int even, const char* device_src, int accumulate,
+                           char* out, int64_t out_len, int64_t* code_size) {
+  if (dim < 1 || dim > DP_DMAX || !device_src) return DP_ERR_ARG;
+  std::string src = pair_source(dim, precision, ntypes, even, rs_for_dim(dim), accumulate, device_src);
+  std::vector<char> code;
+  std::string lg;
+  int rc = compile_cached(src, code, lg);
+  if (rc != DP_OK) return rc;
+  if (code_size) *code_size = (int64_t)code.size();
+  if (out && out_len >= (int64_t)code.size()) std::memcpy(out, code.data(), code.size());
+  return DP_OK;
+}
+
+DP_API const char* dp_last_error(const dp_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+DP_API int dp_create(dp_ctx** out, int device, int dim, int64_t n_total, int n_replicas, int precision) {
+  if (!out) return DP_ERR_ARG;
+  *out = nullptr;
+  if (dim < 1 || dim > DP_DMAX) {
+    g_create_error = "dimension must be in 1..8";
+    return DP_ERR_ARG;
+  }
+  if (n_total < 1 || n_replicas < 1 || n_replicas > n_total) {
+    g_create_error = "need n_total >= n_replicas >= 1";
+    return DP_ERR_ARG;
+  }
+  if (precision != DP_FP64 && precision != DP_MIXED) {
+    g_create_error = "precision must be DP_FP64 or DP_MIXED";
+    return DP_ERR_ARG;
+  }
+  const int sb = 2 * dim;
+  if (sb < 32 && (uint64_t)n_total > ((uint64_t)1 << (32 - sb))) {
+    g_create_error = "too many particles for the neighbour-list entry format at this dimension";
+    return DP_ERR_ARG;
+  }
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) {
+    g_create_error = std::string("no HIP device available: ") + hipGetErrorString(e);
+    return DP_ERR_HIP;
+  }
+  if (device < 0 || device >= ndev) {
+    g_create_error = "device index out of range";
+    return DP_ERR_ARG;
+  }
+  dp_ctx* ctx = new dp_ctx();
+  ctx->dev = device;
+  ctx->D = dim;
+  ctx->N = n_total;
+  ctx->Ns = (n_total + 63) / 64 * 64;
+  ctx->R = n_replicas;
+  ctx->mode = precision;
+  ctx->rs = rs_for_dim(dim);
+  std::memset(&ctx->tab_h, 0, sizeof(ctx->tab_h));
+  auto bail = [&](int rc) {
+    g_create_error = ctx->err;
+    dp_destroy(ctx);
+    return rc;
+  };
+#define CCHK(expr)                 \
+  do {                             \
+    int rc_ = (expr);              \
+    if (rc_ != DP_OK) return bail(rc_); \
+  } while (0)
+  if (hipSetDevice(device) != hipSuccess) return bail(ctx->fail(DP_ERR_HIP, "hipSetDevice failed"));
+  if (hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess)
+    return bail(ctx->fail(DP_ERR_HIP, "hipStreamCreate failed"));
+  const i64 N = n_total, D = dim;
+  CCHK(dalloc(ctx, ctx->reps_d, (size_t)n_replicas));
+  CCHK(dalloc(ctx, ctx->tab_d, 1));
+  CCHK(dalloc(ctx, ctx->ctl_d, 1));
+  CCHK(dalloc(ctx, ctx->dispmax_d, (size_t)n_replicas));
+  CCHK(dalloc(ctx, ctx->nlupd_d, (size_t)n_replicas));
+  CCHK(dalloc(ctx, ctx->red, (size_t)n_replicas));
+  CCHK(dalloc(ctx, ctx->x[0], (size_t)(N * D)));
+  CCHK(dalloc(ctx, ctx->x[1], (size_t)(N * D)));
+  if (precision == DP_MIXED) {
+    CCHK(dalloc(ctx, ctx->xs[0], (size_t)(N * ctx->rs)));
+    CCHK(dalloc(ctx, ctx->xs[1], (size_t)(N * ctx->rs)));
+  }
+  CCHK(dalloc(ctx, ctx->v, (size_t)(N * D)));
+  CCHK(dalloc(ctx, ctx->beta, (size_t)(N * D)));
+  CCHK(dalloc(ctx, ctx->mass, (size_t)N));
+  CCHK(dalloc(ctx, ctx->xref, (size_t)(N * D)));
+  CCHK(dalloc(ctx, ctx->F, (size_t)(N * D)));
+  CCHK(dalloc(ctx, ctx->E, (size_t)N));
+  CCHK(dalloc(ctx, ctx->W, (size_t)N));
+  CCHK(dalloc(ctx, ctx->LAP, (size_t)N));
+  CCHK(dalloc(ctx, ctx->xi, (size_t)(N * D)));
+  CCHK(dalloc(ctx, ctx->img, (size_t)(N * D)));
+  CCHK(dalloc(ctx, ctx->type, (size_t)N));
+  CCHK(dalloc(ctx, ctx->orig, (size_t)N));
+  CCHK(dalloc(ctx, ctx->rep, (size_t)N));
+  CCHK(dalloc(ctx, ctx->cellpack, (size_t)N));
+  CCHK(dalloc(ctx, ctx->cell_of, (size_t)N));
+  CCHK(dalloc(ctx, ctx->rank, (size_t)N));
+  CCHK(dalloc(ctx, ctx->tmp, (size_t)N));
+  CCHK(dalloc(ctx, ctx->perm, (size_t)N));
+  CCHK(dalloc(ctx, ctx->nlcnt, (size_t)N));
+  CCHK(dalloc(ctx, ctx->scratch, (size_t)(N * (3 * D * 8 + 8 + D * 4 + 12))));
+  CCHK(dalloc(ctx, ctx->stage, (size_t)(N * D * 8)));
+  if (hipHostMalloc((void**)&ctx->ctl_h, sizeof(DpCtl), hipHostMallocDefault) != hipSuccess)
+    return bail(ctx->fail(DP_ERR_HIP, "hipHostMalloc failed"));
+  // defaults: identity order, one replica covering everything, zero state
+  {
+    std::vector<i32> ident((size_t)N);
+    for (i64 i = 0; i < N; ++i) ident[i] = (i32)i;
+    if (hipMemcpy(ctx->orig, ident.data(), N * 4, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(ctx->fail(DP_ERR_HIP, "hipMemcpy failed"));
+    std::vector<i32> zi((size_t)(N * D), 0);
+    std::vector<double> zd((size_t)(N * D), 0.0);
+    std::vector<double> ones((size_t)N, 1.0);
+    bool ok = hipMemcpy(ctx->img, zi.data(), N * D * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(ctx->type, zi.data(), N * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(ctx->x[0], zd.data(), N * D * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(ctx->v, zd.data(), N * D * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(ctx->beta, zd.data(), N * D * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(ctx->xref, zd.data(), N * D * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(ctx->mass, ones.data(), N * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(ctx->nlcnt, zi.data(), N * 4, hipMemcpyHostToDevice) == hipSuccess;
+    DpCtl c;
+    std::memset(&c, 0, sizeof(c));
+    ok = ok && hipMemcpy(ctx->ctl_d, &c, sizeof(c), hipMemcpyHostToDevice) == hipSuccess;
+    std::vector<u64> z64((size_t)n_replicas, 0);
+    ok = ok && hipMemcpy(ctx->dispmax_d, z64.data(), 8 * n_replicas, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(ctx->nlupd_d, z64.data(), 8 * n_replicas, hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) return bail(ctx->fail(DP_ERR_HIP, "initialising device buffers failed"));
+  }
+  ctx->offsets = {0, n_total};
+  {
+    // every particle in replica 0 until dp_set_replicas says otherwise
+    std::vector<i32> r0((size_t)N, 0);
+    if (hipMemcpy(ctx->rep, r0.data(), N * 4, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(ctx->fail(DP_ERR_HIP, "hipMemcpy failed"));
+  }
+#undef CCHK
+  *out = ctx;
+  return DP_OK;
+}
+
+DP_API int dp_destroy(dp_ctx* ctx) {
+  if (!ctx) return DP_OK;
+  (void)hipSetDevice(ctx->dev);
+  if (ctx->st) (void)hipStreamSynchronize(ctx->st);
+  release_graph(ctx);
+  for (auto& ev : ctx->ev_pending) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+  for (auto& ev : ctx->ev_free) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+  if (ctx->mod) (void)hipModuleUnload(ctx->mod);
+  dfree(ctx->reps_d); dfree(ctx->tab_d); dfree(ctx->ctl_d); dfree(ctx->dispmax_d); dfree(ctx->nlupd_d);
+  dfree(ctx->red); dfree(ctx->x[0]); dfree(ctx->x[1]); dfree(ctx->xs[0]); dfree(ctx->xs[1]);
+  dfree(ctx->v); dfree(ctx->beta); dfree(ctx->mass); dfree(ctx->xref); dfree(ctx->F); dfree(ctx->E);
+  dfree(ctx->W); dfree(ctx->LAP); dfree(ctx->xi); dfree(ctx->img); dfree(ctx->type); dfree(ctx->orig);
+  dfree(ctx->rep); dfree(ctx->cellpack); dfree(ctx->cell_of); dfree(ctx->rank); dfree(ctx->tmp);
+  dfree(ctx->perm); dfree(ctx->nlcnt); dfree(ctx->ccount); dfree(ctx->cstart); dfree(ctx->tiles);
+  dfree(ctx->nl); dfree(ctx->scratch); dfree(ctx->stage);
+  if (ctx->ctl_h) (void)hipHostFree(ctx->ctl_h);
+  if (ctx->st) (void)hipStreamDestroy(ctx->st);
+  delete ctx;
+  return DP_OK;
+}
+
+DP_API int dp_set_replicas(dp_ctx* ctx, const int64_t* offsets, const double* box) {
+  if (!ctx || !offsets || !box) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  const int R = ctx->R;
+  if (offsets[0] != 0 || offsets[R] != ctx->N) return ctx->fail(DP_ERR_ARG, "offsets must run from 0 to N");
+  for (int r = 0; r < R; ++r)
+    if (offsets[r + 1] <= offsets[r]) return ctx->fail(DP_ERR_ARG, "every replica needs at least one particle");
+  for (int k = 0; k < R * ctx->D; ++k)
+    if (!(box[k] > 0.0) || !std::isfinite(box[k])) return ctx->fail(DP_ERR_ARG, "box lengths must be positive");
+  const bool same_offsets = std::equal(ctx->offsets.begin(), ctx->offsets.end(), offsets) &&
+                            (int)ctx->offsets.size() == R + 1;
+  ctx->offsets.assign(offsets, offsets + R + 1);
+  ctx->box.assign(box, box + (size_t)R * ctx->D);
+  ctx->have_box = true;
+  ctx->reps_dirty = true;
+  ctx->nl_built = false;  // box changed: the list is rebuilt at the next use
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  if (!same_offsets) {
+    std::vector<i32> rv((size_t)ctx->N);
+    for (int r = 0; r < R; ++r)
+      for (i64 i = offsets[r]; i < offsets[r + 1]; ++i) rv[i] = r;
+    HIPCHK(hipMemcpy(ctx->rep, rv.data(), ctx->N * 4, hipMemcpyHostToDevice));
+  }
+  release_graph(ctx);
+  return DP_OK;
+}
+
+DP_API int dp_set_integrator(dp_ctx* ctx, const double* dt, const double* temperature, const double* damping_time) {
+  if (!ctx || !dt || !temperature || !damping_time) return DP_ERR_ARG;
+  const int R = ctx->R;
+  ctx->dt.assign(dt, dt + R);
+  ctx->T.assign(temperature, temperature + R);
+  ctx->tau.assign(damping_time, damping_time + R);
+  ctx->have_integ = true;
+  ctx->reps_dirty = true;
+  return DP_OK;
+}
+
+DP_API int dp_set_neighbor_params(dp_ctx* ctx, const double* cutoff, const double* skin, int max_neighbors,
+                                  int order) {
+  if (!ctx || !cutoff || !skin) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  if (max_neighbors < 1) return ctx->fail(DP_ERR_ARG, "max_number_of_neighbors must be >= 1");
+  if (order != DP_ORDER_X_FASTEST && order != DP_ORDER_LAST_FASTEST) return ctx->fail(DP_ERR_ARG, "bad order");
+  const int R = ctx->R;
+  for (int r = 0; r < R; ++r)
+    if (std::isnan(cutoff[r]) || std::isnan(skin[r])) return ctx->fail(DP_ERR_ARG, "cutoff/skin must not be NaN");
+  ctx->cutoff.assign(cutoff, cutoff + R);
+  ctx->skin.assign(skin, skin + R);
+  ctx->order = order;
+  if (max_neighbors != ctx->K || !ctx->nl) {
+    const size_t bytes = (size_t)max_neighbors * (size_t)ctx->Ns * 4;
+    dfree(ctx->nl);
+    if (hipMalloc((void**)&ctx->nl, bytes) != hipSuccess) {
+      ctx->K = 0;
+      return ctx->fail(DP_ERR_HIP, "cannot allocate neighbour list of %zu bytes", bytes);
+    }
+    ctx->K = max_neighbors;
+    release_graph(ctx);
+  }
+  ctx->have_nl_params = true;
+  ctx->reps_dirty = true;
+  ctx->nl_built = false;
+  return DP_OK;
+}
+
+DP_API int dp_load_potential(dp_ctx* ctx, const char* device_src, int even, double r_cut, double u_rc, int ntypes,
+                             const double* sigma, const double* eps, int accumulate) {
+  if (!ctx || !device_src || !sigma || !eps) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  if (ntypes < 1 || ntypes > DP_MAXTYPES) return ctx->fail(DP_ERR_ARG, "ntypes must be in 1..%d", DP_MAXTYPES);
+  if (accumulate != 0 && accumulate != 1) return ctx->fail(DP_ERR_ARG, "accumulate must be 0 or 1");
+  for (int k = 0; k < ntypes * ntypes; ++k)
+    if (!(sigma[k] > 0.0)) return ctx->fail(DP_ERR_ARG, "sigma must be positive");
+  std::string src = pair_source(ctx->D, ctx->mode, ntypes, even, ctx->rs, accumulate, device_src);
+  std::vector<char> code;
+  std::string log;
+  if (compile_cached(src, code, log) != DP_OK) return ctx->fail(DP_ERR_RTC, "hiprtc compile failed: %s", log.c_str());
+  release_graph(ctx);
+  if (ctx->mod) {
+    (void)hipModuleUnload(ctx->mod);
+    ctx->mod = nullptr;
+  }
+  HIPCHK(hipModuleLoadData(&ctx->mod, code.data()));
+  HIPCHK(hipModuleGetFunction(&ctx->f_force, ctx->mod, "dp_pair_force"));
+  HIPCHK(hipModuleGetFunction(&ctx->f_obs, ctx->mod, "dp_pair_obs"));
+  HIPCHK(hipModuleGetFunction(&ctx->f_gjf, ctx->mod, "dp_step_gjf"));
+  HIPCHK(hipModuleGetFunction(&ctx->f_lf, ctx->mod, "dp_step_lf"));
+  HIPCHK(hipModuleGetFunction(&ctx->f_int_gjf, ctx->mod, "dp_integrate_gjf"));
+  HIPCHK(hipModuleGetFunction(&ctx->f_int_lf, ctx->mod, "dp_integrate_lf"));
+  // tables
+  DpPairTab& T = ctx->tab_h;
+  std::memset(&T, 0, sizeof(T));
+  ctx->ntypes = ntypes;
+  for (int a = 0; a < ntypes; ++a)
+    for (int b = 0; b < ntypes; ++b) {
+      const int k = a * ntypes + b;
+      const double s = sigma[k], e = eps[k];
+      T.sigma[k] = s;
+      T.eps[k] = e;
+      T.cut_thresh[k] = cut_threshold(s, r_cut);
+      T.inv_sigma[k] = 1.0 / s;
+      T.inv_sigma2[k] = 1.0 / (s * s);
+      T.sigma_f[k] = (float)s;
+      T.eps_f[k] = (float)e;
+      T.eps_over_sigma_f[k] = (float)(e / s);
+      T.inv_sigma_f[k] = (float)(1.0 / s);
+      T.inv_sigma2_f[k] = (float)(1.0 / (s * s));
+    }
+  T.r_cut = r_cut;
+  T.u_rc = u_rc;
+  ctx->pot_rcut = r_cut;
+  ctx->have_pot = true;
+  ctx->tab_dirty = true;
+  return DP_OK;
+}
+
+namespace {
+int field_info(dp_ctx* ctx, int field, void** dev, size_t* elem, int* width) {
+  switch (field) {
+    case DP_FIELD_POSITIONS: *dev = ctx->x[ctx->cur]; *elem = 8; *width = ctx->D; return DP_OK;
+    case DP_FIELD_VELOCITIES: *dev = ctx->v; *elem = 8; *width = ctx->D; return DP_OK;
+    case DP_FIELD_BETAS: *dev = ctx->beta; *elem = 8; *width = ctx->D; return DP_OK;
+    case DP_FIELD_IMAGES: *dev = ctx->img; *elem = 4; *width = ctx->D; return DP_OK;
+    case DP_FIELD_MASSES: *dev = ctx->mass; *elem = 8; *width = 1; return DP_OK;
+    case DP_FIELD_TYPES: *dev = ctx->type; *elem = 4; *width = 1; return DP_OK;
+    case DP_FIELD_XREF: *dev = ctx->xref; *elem = 8; *width = ctx->D; return DP_OK;
+  }
+  return ctx->fail(DP_ERR_ARG, "unknown field %d", field);
+}
+}  // namespace
+
+DP_API int dp_set_field(dp_ctx* ctx, int field, const void* src) {
+  if (!ctx || !src) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  void* dev;
+  size_t elem;
+  int width;
+  CHK(field_info(ctx, field, &dev, &elem, &width));
+  const size_t bytes = (size_t)ctx->N * width * elem;
+  HIPCHK(hipMemcpyAsync(ctx->stage, src, bytes, hipMemcpyDefault, ctx->st));
+  const int g = grid_for(ctx->N);
+  if (elem == 8)
+    k_from_orig<double><<<g, DP_BLOCK, 0, ctx->st>>>((const double*)ctx->stage, (double*)dev, ctx->orig, ctx->N, width);
+  else
+    k_from_orig<i32><<<g, DP_BLOCK, 0, ctx->st>>>((const i32*)ctx->stage, (i32*)dev, ctx->orig, ctx->N, width);
+  HIPCHK(hipGetLastError());
+  if (field == DP_FIELD_TYPES && ctx->have_pot) {
+    // type indices are used to address the sigma/epsilon tables
+    std::vector<i32> t((size_t)ctx->N);
+    HIPCHK(hipMemcpyAsync(t.data(), ctx->stage, ctx->N * 4, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    for (i64 i = 0; i < ctx->N; ++i)
+      if (t[i] < 0 || t[i] >= ctx->ntypes) return ctx->fail(DP_ERR_ARG, "particle type %d out of range", t[i]);
+  }
+  if (field == DP_FIELD_POSITIONS || field == DP_FIELD_XREF) ctx->pos_dirty = true;
+  if (field == DP_FIELD_POSITIONS || field == DP_FIELD_TYPES) CHK(launch_records(ctx));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return DP_OK;
+}
+
+DP_API int dp_get_field(dp_ctx* ctx, int field, void* dst) {
+  if (!ctx || !dst) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  void* dev;
+  size_t elem;
+  int width;
+  CHK(field_info(ctx, field, &dev, &elem, &width));
+  const size_t bytes = (size_t)ctx->N * width * elem;
+  const int g = grid_for(ctx->N);
+  if (elem == 8)
+    k_to_orig<double><<<g, DP_BLOCK, 0, ctx->st>>>((const double*)dev, (double*)ctx->stage, ctx->orig, ctx->N, width);
+  else
+    k_to_orig<i32><<<g, DP_BLOCK, 0, ctx->st>>>((const i32*)dev, (i32*)ctx->stage, ctx->orig, ctx->N, width);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dst, ctx->stage, bytes, hipMemcpyDefault, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return DP_OK;
+}
+
+DP_API int dp_build_neighbor_list(dp_ctx* ctx) {
+  if (!ctx) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  CHK(refresh_reps(ctx));
+  k_request_rebuild<<<1, 1, 0, ctx->st>>>(ctx->ctl_d);
+  CHK(launch_prologue(ctx, 0));
+  CHK(launch_rebuild(ctx));
+  ctx->nl_built = true;
+  ctx->pos_dirty = false;
+  return check_ctl(ctx);
+}
+
+DP_API int dp_update_neighbor_list(dp_ctx* ctx, int* rebuilt) {
+  if (!ctx) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  CHK(refresh_reps(ctx));
+  if (!ctx->nl_built) {
+    k_request_rebuild<<<1, 1, 0, ctx->st>>>(ctx->ctl_d);
+    ctx->nl_built = true;
+  } else {
+    CHK(launch_skin_check(ctx));
+  }
+  ctx->pos_dirty = false;
+  CHK(launch_prologue(ctx, 0));
+  CHK(launch_rebuild(ctx));
+  CHK(check_ctl(ctx));
+  if (rebuilt) *rebuilt = ctx->ctl_h->rebuild;
+  return DP_OK;
+}
+
+DP_API int dp_compute(dp_ctx* ctx, double* forces, double* energy, double* virial, double* laplacian) {
+  if (!ctx) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  CHK(prepare(ctx));
+  int rebuilt = 0;
+  CHK(dp_update_neighbor_list(ctx, &rebuilt));
+  DpPairArgs a = pair_args(ctx, ctx->cur);
+  const bool obs = energy || virial || laplacian;
+  a.F = ctx->F;
+  if (obs) {
+    a.E = energy ? ctx->E : nullptr;
+    a.W = virial ? ctx->W : nullptr;
+    a.LAP = laplacian ? ctx->LAP : nullptr;
+    CHK(launch_pair(ctx, ctx->f_obs, a));
+  } else {
+    CHK(launch_pair(ctx, ctx->f_force, a));
+  }
+  const int g = grid_for(ctx->N);
+  if (forces) {
+    k_to_orig<double><<<g, DP_BLOCK, 0, ctx->st>>>(ctx->F, (double*)ctx->stage, ctx->orig, ctx->N, ctx->D);
+    HIPCHK(hipMemcpyAsync(forces, ctx->stage, (size_t)ctx->N * ctx->D * 8, hipMemcpyDefault, ctx->st));
+  }
+  std::vector<double> tmp((size_t)ctx->R);
+  if (energy) {
+    k_reduce_rep<<<ctx->R, DP_BLOCK, 0, ctx->st>>>(ctx->E, ctx->reps_d, ctx->red);
+    HIPCHK(hipMemcpyAsync(tmp.data(), ctx->red, 8 * ctx->R, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    for (int r = 0; r < ctx->R; ++r) energy[r] = tmp[r] / 2;  // potential.py:100
+  }
+  if (virial) {
+    k_reduce_rep<<<ctx->R, DP_BLOCK, 0, ctx->st>>>(ctx->W, ctx->reps_d, ctx->red);
+    HIPCHK(hipMemcpyAsync(tmp.data(), ctx->red, 8 * ctx->R, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    for (int r = 0; r < ctx->R; ++r) virial[r] = tmp[r] / 2 / ctx->D;  // each pair counted twice; potential.py:279
+  }
+  if (laplacian) {
+    k_to_orig<double><<<g, DP_BLOCK, 0, ctx->st>>>(ctx->LAP, (double*)ctx->stage, ctx->orig, ctx->N, 1);
+    HIPCHK(hipMemcpyAsync(laplacian, ctx->stage, (size_t)ctx->N * 8, hipMemcpyDefault, ctx->st));
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return DP_OK;
+}
+
+DP_API int dp_step(dp_ctx* ctx, int64_t nsteps, int integrator, const double* xi) {
+  if (!ctx) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  if (integrator != DP_INTEGRATOR_GJF && integrator != DP_INTEGRATOR_LEAPFROG)
+    return ctx->fail(DP_ERR_ARG, "unknown integrator %d", integrator);
+  if (nsteps < 0) return ctx->fail(DP_ERR_ARG, "nsteps must be >= 0");
+  if (nsteps == 0) return DP_OK;
+  if (xi && nsteps != 1) return ctx->fail(DP_ERR_ARG, "injected noise requires nsteps == 1");
+  if (!ctx->have_integ) return ctx->fail(DP_ERR_STATE, "dp_set_integrator has not been called");
+  CHK(prepare(ctx));
+  CHK(pre_step_checks(ctx));
+  if (xi) {
+    HIPCHK(hipMemcpyAsync(ctx->xi, xi, (size_t)ctx->N * ctx->D * 8, hipMemcpyDefault, ctx->st));
+    CHK(launch_one_step(ctx, integrator, ctx->xi));
+    CHK(resolve_timing(ctx));
+    return check_ctl(ctx);
+  }
+  int64_t left = nsteps;
+  const bool use_graph = ctx->graph_block > 0 && !ctx->timing && nsteps >= 2 * (int64_t)ctx->graph_block;
+  if (use_graph) {
+    if (ctx->cur != 0) {
+      CHK(launch_one_step(ctx, integrator, nullptr));
+      --left;
+    }
+    if (ctx->graph_dirty || !ctx->gexec || ctx->gexec_integ != integrator) CHK(capture_graph(ctx, integrator));
+    while (left >= ctx->graph_block) {
+      HIPCHK(hipGraphLaunch(ctx->gexec, ctx->st));
+      left -= ctx->graph_block;
+    }
+  }
+  while (left > 0) {
+    CHK(launch_one_step(ctx, integrator, nullptr));
+    --left;
+    if (ctx->timing && ctx->ev_pending.size() >= 256) CHK(resolve_timing(ctx));
+  }
+  CHK(resolve_timing(ctx));
+  return check_ctl(ctx);
+}
+
+DP_API int dp_integrate(dp_ctx* ctx, const double* forces, int integrator, const double* xi) {
+  if (!ctx || !forces) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  if (integrator != DP_INTEGRATOR_GJF && integrator != DP_INTEGRATOR_LEAPFROG)
+    return ctx->fail(DP_ERR_ARG, "unknown integrator %d", integrator);
+  if (!ctx->have_integ) return ctx->fail(DP_ERR_STATE, "dp_set_integrator has not been called");
+  CHK(prepare(ctx));
+  HIPCHK(hipMemcpyAsync(ctx->stage, forces, (size_t)ctx->N * ctx->D * 8, hipMemcpyDefault, ctx->st));
+  k_from_orig<double><<<grid_for(ctx->N), DP_BLOCK, 0, ctx->st>>>((const double*)ctx->stage, ctx->F, ctx->orig,
+                                                                  ctx->N, ctx->D);
+  if (xi) HIPCHK(hipMemcpyAsync(ctx->xi, xi, (size_t)ctx->N * ctx->D * 8, hipMemcpyDefault, ctx->st));
+  k_advance_step<<<1, 1, 0, ctx->st>>>(ctx->ctl_d);
+  DpPairArgs a = pair_args(ctx, ctx->cur);
+  a.F = ctx->F;
+  a.xi = xi ? ctx->xi : nullptr;
+  CHK(launch_pair(ctx, integrator == DP_INTEGRATOR_GJF ? ctx->f_int_gjf : ctx->f_int_lf, a));
+  ctx->cur = 1 - ctx->cur;
+  ctx->pos_dirty = true;  // the list's skin check must see the new positions
+  CHK(launch_records(ctx));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return DP_OK;
+}
+
+DP_API int dp_set_seed(dp_ctx* ctx, uint64_t seed, uint64_t step_counter) {
+  if (!ctx) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  ctx->seed = seed;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  HIPCHK(hipMemcpy(ctx->ctl_h, ctx->ctl_d, sizeof(DpCtl), hipMemcpyDeviceToHost));
+  ctx->ctl_h->step_counter = step_counter;
+  HIPCHK(hipMemcpy(ctx->ctl_d, ctx->ctl_h, sizeof(DpCtl), hipMemcpyHostToDevice));
+  release_graph(ctx);  // the seed is a kernel argument
+  return DP_OK;
+}
+
+DP_API int dp_get_neighbor_list(dp_ctx* ctx, int32_t* out) {
+  if (!ctx || !out) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  if (!ctx->nl_built) CHK(dp_build_neighbor_list(ctx));
+  const size_t bytes = (size_t)ctx->N * ctx->K * 4;
+  i32* tmpbuf = nullptr;
+  HIPCHK(hipMalloc((void**)&tmpbuf, bytes));
+  k_nl_export<<<grid_for(ctx->N), DP_BLOCK, 0, ctx->st>>>(ctx->nl, ctx->nlcnt, ctx->orig, ctx->N, ctx->Ns, ctx->K,
+                                                          2 * ctx->D, tmpbuf);
+  hipError_t e = hipMemcpyAsync(out, tmpbuf, bytes, hipMemcpyDefault, ctx->st);
+  hipError_t e2 = hipStreamSynchronize(ctx->st);
+  (void)hipFree(tmpbuf);
+  if (e != hipSuccess || e2 != hipSuccess) return ctx->fail(DP_ERR_HIP, "neighbour-list export failed");
+  return DP_OK;
+}
+
+DP_API int dp_get_neighbor_counts(dp_ctx* ctx, int32_t* out) {
+  if (!ctx || !out) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  if (!ctx->nl_built) CHK(dp_build_neighbor_list(ctx));
+  k_to_orig<i32><<<grid_for(ctx->N), DP_BLOCK, 0, ctx->st>>>(ctx->nlcnt, (i32*)ctx->stage, ctx->orig, ctx->N, 1);
+  HIPCHK(hipMemcpyAsync(out, ctx->stage, (size_t)ctx->N * 4, hipMemcpyDefault, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return DP_OK;
+}
+
+DP_API int dp_get_stats(dp_ctx* ctx, int64_t* nl_updates, int64_t* steps, int32_t* max_count) {
+  if (!ctx) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  if (nl_updates) HIPCHK(hipMemcpy(nl_updates, ctx->nlupd_d, 8 * ctx->R, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(ctx->ctl_h, ctx->ctl_d, sizeof(DpCtl), hipMemcpyDeviceToHost));
+  if (steps) *steps = (int64_t)ctx->ctl_h->step_counter;
+  if (max_count) *max_count = ctx->ctl_h->max_count;
+  return DP_OK;
+}
+
+DP_API int dp_set_graph_block(dp_ctx* ctx, int steps) {
+  if (!ctx || steps < 0) return DP_ERR_ARG;
+  if (steps % 2) ++steps;  // the block must return to the same buffer parity
+  ctx->graph_block = steps;
+  release_graph(ctx);
+  return DP_OK;
+}
+
+DP_API int dp_set_kernel_timing(dp_ctx* ctx, int enable) {
+  if (!ctx) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  CHK(resolve_timing(ctx));
+  ctx->timing = enable != 0;
+  ctx->time_ms = 0.0;
+  ctx->time_launches = 0;
+  return DP_OK;
+}
+
+DP_API int dp_get_kernel_time(dp_ctx* ctx, double* total_ms, int64_t* launches) {
+  if (!ctx) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  CHK(resolve_timing(ctx));
+  if (total_ms) *total_ms = ctx->time_ms;
+  if (launches) *launches = ctx->time_launches;
+  return DP_OK;
+}
+
+DP_API int dp_sync(dp_ctx* ctx) {
+  if (!ctx) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return DP_OK;
+}
+
+DP_API int dp_get_stream(dp_ctx* ctx, void** stream) {
+  if (!ctx || !stream) return DP_ERR_ARG;
+  *stream = (void*)ctx->st;
+  return DP_OK;
+}

@@ -1,0 +1,225 @@
+"""Engine: one libdompap_hip context (one or more replicas on one GPU).
+
+Arrays cross the boundary in the reference's layout (row n = particle n).
+Inputs may be NumPy arrays (host) or torch device tensors (their data_ptr()
+is handed to the library; it copies device-to-device).  Outputs are NumPy
+arrays unless an ``out`` tensor is given.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .codegen import PotentialCode
+
+
+def _ptr(a):
+    """Raw pointer of a contiguous NumPy array or torch tensor."""
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return ctypes.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _as_f64(a, shape):
+    if hasattr(a, "data_ptr"):
+        import torch  # noqa: F401  (only for device tensors)
+        if a.dtype != __import__("torch").float64 or tuple(a.shape) != tuple(shape):
+            raise ValueError(f"expected float64 tensor of shape {shape}")
+        return a.contiguous()
+    arr = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    if arr.shape != tuple(shape):
+        arr = arr.reshape(shape)
+    return arr
+
+
+def _as_i32(a, shape):
+    if hasattr(a, "data_ptr"):
+        if tuple(a.shape) != tuple(shape):
+            raise ValueError(f"expected tensor of shape {shape}")
+        return a.contiguous()
+    arr = np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+    if arr.shape != tuple(shape):
+        arr = arr.reshape(shape)
+    return arr
+
+
+class Engine:
+    """Device state and kernels for ``n_total`` particles in ``n_replicas``
+    independent systems of dimension ``dim``."""
+
+    def __init__(self, dim: int, n_total: int, n_replicas: int = 1, precision: str = "fp64", device: int = 0):
+        self.lib = _lib.load()
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.dim = int(dim)
+        self.n = int(n_total)
+        self.n_replicas = int(n_replicas)
+        self.precision = precision
+        ctx = ctypes.c_void_p()
+        rc = self.lib.dp_create(ctypes.byref(ctx), int(device), self.dim, self.n, self.n_replicas,
+                                _lib.DP_MIXED if precision == "mixed" else _lib.DP_FP64)
+        _lib.check(rc, None)
+        self.ctx = ctx
+        self.max_neighbors = 0
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.dp_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        _lib.check(rc, self.ctx)
+
+    # -- parameters -------------------------------------------------------
+    def _per_rep(self, v):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), (self.n_replicas,)))
+        return a
+
+    def set_replicas(self, offsets: Sequence[int], box) -> None:
+        off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+        bx = np.ascontiguousarray(np.asarray(box, dtype=np.float64).reshape(self.n_replicas, self.dim))
+        self._chk(self.lib.dp_set_replicas(self.ctx, _ptr(off), _ptr(bx)))
+
+    def set_box(self, box) -> None:
+        """Single-replica convenience."""
+        self.set_replicas([0, self.n], np.asarray(box, dtype=np.float64).reshape(1, self.dim))
+
+    def set_integrator(self, dt, temperature, damping_time) -> None:
+        a, b, c = self._per_rep(dt), self._per_rep(temperature), self._per_rep(damping_time)
+        self._chk(self.lib.dp_set_integrator(self.ctx, _ptr(a), _ptr(b), _ptr(c)))
+
+    def set_neighbor_params(self, cutoff, skin, max_neighbors: int, order: int = _lib.DP_ORDER_X_FASTEST) -> None:
+        a, b = self._per_rep(cutoff), self._per_rep(skin)
+        self._chk(self.lib.dp_set_neighbor_params(self.ctx, _ptr(a), _ptr(b), int(max_neighbors), int(order)))
+        self.max_neighbors = int(max_neighbors)
+
+    def load_potential(self, code: PotentialCode, r_cut: float, u_rc: float, sigma, epsilon,
+                       accumulate: str = "fp64") -> None:
+        sig = np.ascontiguousarray(np.atleast_2d(np.asarray(sigma, dtype=np.float64)))
+        eps = np.ascontiguousarray(np.atleast_2d(np.asarray(epsilon, dtype=np.float64)))
+        nt = sig.shape[0]
+        if sig.shape != (nt, nt) or eps.shape != (nt, nt):
+            raise ValueError("sigma/epsilon tables must be square and of equal size")
+        acc = {"fp64": 0, "kahan": 1}[accumulate]
+        rc = self.lib.dp_load_potential(self.ctx, code.source.encode(), int(code.even), float(r_cut),
+                                        float(u_rc), nt, _ptr(sig), _ptr(eps), acc)
+        self._chk(rc)
+
+    def set_seed(self, seed: int, step_counter: int = 0) -> None:
+        self._chk(self.lib.dp_set_seed(self.ctx, int(seed) & (2**64 - 1), int(step_counter)))
+
+    def set_graph_block(self, steps: int) -> None:
+        self._chk(self.lib.dp_set_graph_block(self.ctx, int(steps)))
+
+    # -- state ------------------------------------------------------------
+    def _field(self, name):
+        code, kind, wide = _lib.FIELDS[name]
+        shape = (self.n, self.dim) if wide else (self.n,)
+        return code, kind, shape
+
+    def set_field(self, name: str, value) -> None:
+        code, kind, shape = self._field(name)
+        arr = _as_f64(value, shape) if kind == "f8" else _as_i32(value, shape)
+        self._chk(self.lib.dp_set_field(self.ctx, code, _ptr(arr)))
+
+    def get_field(self, name: str, out=None):
+        code, kind, shape = self._field(name)
+        if out is None:
+            out = np.empty(shape, dtype=np.float64 if kind == "f8" else np.int32)
+        self._chk(self.lib.dp_get_field(self.ctx, code, _ptr(out)))
+        return out
+
+    # -- neighbour list ---------------------------------------------------
+    def build_neighbor_list(self) -> None:
+        self._chk(self.lib.dp_build_neighbor_list(self.ctx))
+
+    def update_neighbor_list(self) -> bool:
+        rb = ctypes.c_int(0)
+        self._chk(self.lib.dp_update_neighbor_list(self.ctx, ctypes.byref(rb)))
+        return bool(rb.value)
+
+    def neighbor_list(self) -> np.ndarray:
+        out = np.empty((self.n, self.max_neighbors), dtype=np.int32)
+        self._chk(self.lib.dp_get_neighbor_list(self.ctx, _ptr(out)))
+        return out
+
+    def neighbor_counts(self) -> np.ndarray:
+        out = np.empty((self.n,), dtype=np.int32)
+        self._chk(self.lib.dp_get_neighbor_counts(self.ctx, _ptr(out)))
+        return out
+
+    def stats(self):
+        upd = np.zeros(self.n_replicas, dtype=np.int64)
+        steps = ctypes.c_int64()
+        mc = ctypes.c_int32()
+        self._chk(self.lib.dp_get_stats(self.ctx, upd.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                        ctypes.byref(steps), ctypes.byref(mc)))
+        return upd, int(steps.value), int(mc.value)
+
+    # -- compute ----------------------------------------------------------
+    def compute(self, forces: bool = True, energy: bool = False, virial: bool = False,
+                laplacian: bool = False, forces_out=None):
+        F = None
+        if forces:
+            F = forces_out if forces_out is not None else np.empty((self.n, self.dim), dtype=np.float64)
+        E = np.empty(self.n_replicas) if energy else None
+        W = np.empty(self.n_replicas) if virial else None
+        LAP = np.empty(self.n) if laplacian else None
+        rc = self.lib.dp_compute(self.ctx,
+                                 _ptr(F) if F is not None else None,
+                                 _ptr(E) if E is not None else None,
+                                 _ptr(W) if W is not None else None,
+                                 _ptr(LAP) if LAP is not None else None)
+        self._chk(rc)
+        return {"forces": F, "energy": E, "virial": W, "laplacian": LAP}
+
+    def step(self, nsteps: int = 1, integrator: str = "gjf", xi=None) -> None:
+        integ = _lib.DP_INTEGRATOR_GJF if integrator == "gjf" else _lib.DP_INTEGRATOR_LEAPFROG
+        xp = None
+        if xi is not None:
+            xi = _as_f64(xi, (self.n, self.dim))
+            xp = _ptr(xi)
+        self._chk(self.lib.dp_step(self.ctx, int(nsteps), integ, xp))
+
+    def integrate(self, forces, integrator: str = "gjf", xi=None) -> None:
+        integ = _lib.DP_INTEGRATOR_GJF if integrator == "gjf" else _lib.DP_INTEGRATOR_LEAPFROG
+        f = _as_f64(forces, (self.n, self.dim))
+        xp = None
+        if xi is not None:
+            xi = _as_f64(xi, (self.n, self.dim))
+            xp = _ptr(xi)
+        self._chk(self.lib.dp_integrate(self.ctx, _ptr(f), integ, xp))
+
+    def sync(self) -> None:
+        self._chk(self.lib.dp_sync(self.ctx))
+
+    def set_kernel_timing(self, enable: bool) -> None:
+        self._chk(self.lib.dp_set_kernel_timing(self.ctx, int(bool(enable))))
+
+    def kernel_time(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self._chk(self.lib.dp_get_kernel_time(self.ctx, ctypes.byref(ms), ctypes.byref(n)))
+        return float(ms.value), int(n.value)
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        self._chk(self.lib.dp_get_stream(self.ctx, ctypes.byref(s)))
+        return int(s.value or 0)
+
+
+def exact_is_finite(x: float) -> bool:
+    return math.isfinite(float(x))
