@@ -329,7 +329,11 @@ class Simulation:
                 if f == "masses":
                     v = np.asarray(v, dtype=np.float64).reshape(-1)
                     if v.shape[0] != n:
-                        raise ValueError(f"Expected shape of masses: {(n, 1)}, got: {np.shape(self._h[f])}")
+                        # only the integrator reads masses; like the reference,
+                        # a stale array is an error only when stepping
+                        self.__dict__["_masses_ok"] = False
+                        continue
+                    self.__dict__["_masses_ok"] = True
                 elif f == "image_positions":
                     v = np.asarray(v).astype(np.int32)
                 else:
@@ -616,11 +620,17 @@ class Simulation:
         wrap_into_box(pos.view(np.ndarray), img.view(np.ndarray), np.asarray(self.box_vectors, dtype=np.float64))
         self._dirty.update(("positions", "image_positions"))
 
+    def _check_masses(self):
+        if not self.__dict__.get("_masses_ok", True):
+            raise ValueError(f"Expected shape of masses: {(self.number_of_particles(), 1)}, "
+                             f"got: {np.shape(self._h['masses'])}")
+
     def step(self, noise=None) -> None:
         """Simulation.py:681-704: skin check, (rebuild), forces, G-JF step,
         wrap -- one fused device sequence.  ``noise`` optionally supplies the
         standard normals [N, D] of integrator.py:30."""
         eng = self._device()
+        self._check_masses()
         eng.step(1, "gjf", xi=noise)
         self._after_device_step()
         self.__dict__["number_of_steps"] = self.number_of_steps + 1
@@ -628,6 +638,7 @@ class Simulation:
     def step_leap_frog(self) -> None:
         """Simulation.py:706-724."""
         eng = self._device()
+        self._check_masses()
         eng.step(1, "leapfrog")
         self._after_device_step()
         self.__dict__["number_of_steps"] = self.number_of_steps + 1
@@ -638,6 +649,7 @@ class Simulation:
         if steps <= 0:
             return
         eng = self._device()
+        self._check_masses()
         eng.step(int(steps), "gjf")
         self._after_device_step()
         self.__dict__["number_of_steps"] = self.number_of_steps + int(steps)
