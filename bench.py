@@ -1,0 +1,297 @@
+"""Benchmark: particle-steps/s of the dompap per-step hot path on one MI355X.
+
+Default workload: 3-D Lennard-Jones, eps = sigma = 1, truncated and shifted at
+r_cut = 2.5, N = 1,048,576 on an FCC 64^3 lattice at density 0.8442, Gaussian
+velocities at T = 0.72 (zero total momentum), Langevin G-JF with friction
+alpha/m = 0.1 (damping time tau = 10), dt = 0.005, neighbour skin 0.5, mixed
+precision.  One step = skin check, rebuild when needed (included in the timed
+region), fused force + integrate kernel.
+
+Prints ONE JSON line (see the "metric"/"value"/"roofline"/"cpu_baseline"
+keys).  Extra keys report the FP64 GPU mode on the same workload, the
+mixed/FP64 ratio, force/energy/virial errors of the mixed mode against the
+FP64 GPU mode and the C oracle, and NVE energy drift.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def fcc_lattice(cells, density):
+    from dompap_amd.positions import generate_positions
+    ucc = np.array([(0, 0, 0), (.5, .5, 0), (.5, 0, .5), (0, .5, .5)], dtype=np.float64)
+    a = (4.0 / density) ** (1.0 / 3.0)
+    x = generate_positions(ucc, (cells,) * 3, (a, a, a))
+    return x, np.full(3, cells * a)
+
+
+def sc_lattice(cells, density, dim=3):
+    from dompap_amd.positions import generate_positions
+    a = (1.0 / density) ** (1.0 / dim)
+    x = generate_positions(np.zeros((1, dim)), (cells,) * dim, (a,) * dim)
+    return x, np.full(dim, cells * a)
+
+
+def gaussian_velocities(rng, n, dim, T, masses):
+    v = rng.normal(0.0, 1.0, size=(n, dim)) * np.sqrt(T / masses.reshape(-1, 1))
+    v -= np.sum(v * masses.reshape(-1, 1), axis=0) / np.sum(masses)
+    return v
+
+
+def workload(name, seed):
+    """Synthetic inputs of the named workload (sizes as in the docstring)."""
+    rng = np.random.default_rng(seed)
+    if name == "lj-1m":
+        x, box = fcc_lattice(64, 0.8442)
+        T, skin = 0.72, 0.5
+        desc = "3D LJ FCC 64^3 N=1048576 rho=0.8442 T=0.72 rc=2.5 skin=0.5 dt=0.005 G-JF tau=10"
+    elif name == "lj-4k":
+        x, box = sc_lattice(16, 0.8)
+        T, skin = 0.8, 0.5
+        desc = "3D LJ SC 16^3 N=4096 rho=0.8 T=0.8 rc=2.5 skin=0.5 dt=0.005 G-JF tau=10"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    n = x.shape[0]
+    m = np.ones(n)
+    v = gaussian_velocities(rng, n, 3, T, m)
+    return dict(name=name, desc=desc, x=x, v=v, m=m, box=box, pot="4*(r**-12-r**-6)", r_cut=2.5, skin=skin,
+                dt=0.005, T=T, tau=10.0, K=256, seed=seed)
+
+
+def make_engine(wl, precision, accumulate="fp64"):
+    from dompap_amd.engine import Engine
+    from dompap_amd.potential import device_potential
+    n, dim = wl["x"].shape
+    eng = Engine(dim, n, 1, precision)
+    eng.set_box(wl["box"])
+    dp = device_potential(wl["pot"], wl["r_cut"])
+    eng.load_potential(dp.code, dp.r_cut, dp.u_rc, [[1.0]], [[1.0]], accumulate)
+    eng.set_neighbor_params(1.0 * wl["r_cut"] + wl["skin"], wl["skin"], wl["K"])
+    eng.set_integrator(wl["dt"], wl["T"], wl["tau"])
+    eng.set_seed(wl["seed"], 0)
+    eng.set_field("positions", wl["x"])
+    eng.set_field("velocities", wl["v"])
+    eng.set_field("masses", wl["m"])
+    eng.build_neighbor_list()
+    return eng
+
+
+def sync_all(eng):
+    eng.sync()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def timed_run(eng, steps, timing=True):
+    upd0, st0, _ = eng.stats()
+    eng.set_kernel_timing(timing)
+    sync_all(eng)
+    t0 = time.perf_counter()
+    eng.step(steps, "gjf")
+    sync_all(eng)
+    t1 = time.perf_counter()
+    ms, launches = eng.kernel_time()
+    eng.set_kernel_timing(False)
+    upd1, _, _ = eng.stats()
+    return t1 - t0, ms, launches, int(upd1[0] - upd0[0])
+
+
+def force_errors(F, Fref):
+    dF = np.linalg.norm(F - Fref, axis=1)
+    Fn = np.linalg.norm(Fref, axis=1)
+    frms = float(np.sqrt(np.mean(Fn ** 2)))
+    big = Fn > 0.1 * frms
+    return {"max_abs_over_frms": float(dF.max() / frms),
+            "max_rel_per_particle": float(np.max(dF[big] / Fn[big])) if big.any() else None,
+            "rms_rel": float(np.sqrt(np.mean(dF ** 2)) / frms)}
+
+
+def nve_drift(eng, steps, every):
+    """Slope of E_tot/N versus time for a friction-free segment; E_tot =
+    U(x_n) + (K(v_{n-1/2}) + K(v_{n+1/2}))/2 (leap-frog velocities)."""
+    n = eng.n
+    m = eng.get_field("masses")
+    ts, es = [], []
+    for k in range(steps // every):
+        eng.step(every - 1, "leapfrog")
+        v0 = eng.get_field("velocities")
+        U = eng.compute(forces=False, energy=True)["energy"][0]
+        eng.step(1, "leapfrog")
+        v1 = eng.get_field("velocities")
+        K = 0.25 * (np.sum(m[:, None] * v0 * v0) + np.sum(m[:, None] * v1 * v1))
+        ts.append((k + 1) * every)
+        es.append((U + K) / n)
+    t = np.asarray(ts, dtype=float)
+    e = np.asarray(es)
+    slope = float(np.polyfit(t, e, 1)[0])
+    return slope, float(e.std())
+
+
+def cpu_baseline(wl, x, rebuild_interval, budget_s=20.0):
+    """C oracle (oracle/dp_oracle.c, FP64, OpenMP) on the host cores: one
+    neighbour-list build (cell search; substitutes for the reference's O(N^2)
+    loop) and a few force + G-JF steps at full N, combined with the GPU run's
+    rebuild interval."""
+    from oracle import c_oracle as C
+    n, dim = x.shape
+    box = wl["box"]
+    cores = C.threads()
+    t0 = time.perf_counter()
+    nl, cnt = C.neighbor_list(x, box, wl["r_cut"] + wl["skin"], wl["K"], "cells")
+    t_nl = time.perf_counter() - t0
+    v = wl["v"].copy()
+    beta = np.zeros_like(v)
+    img = np.zeros(x.shape, np.int32)
+    steps = 0
+    t_steps = 0.0
+    xx = x.copy()
+    F0 = None
+    while steps < 2 or (steps < 10 and t_steps < budget_s / 2):
+        t0 = time.perf_counter()
+        F = C.pair(xx, box, nl, "lj", wl["r_cut"])["forces"]
+        if F0 is None:
+            F0 = F
+        xi = C.normals(n, dim, 1, steps)
+        xx, v, beta, img = C.step(xx, v, beta, F, wl["m"], wl["dt"], wl["T"], wl["tau"], xi, box, img)
+        t_steps += time.perf_counter() - t0
+        steps += 1
+    per_step = t_steps / steps + t_nl / max(rebuild_interval, 1.0)
+    return {"value": n / per_step, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": f"C oracle FP64 (OpenMP) on N={n}: 1 cell-list neighbour build ({t_nl:.2f}s) + "
+                      f"{steps} force+G-JF steps ({t_steps / steps:.3f}s each), rebuild every "
+                      f"{rebuild_interval:.1f} steps as measured on the GPU"}, F0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--workload", default="lj-1m")
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--accumulate", default="fp64", choices=["fp64", "kahan"])
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-nve", action="store_true", help="skip the NVE drift segment")
+    ap.add_argument("--nve-steps", type=int, default=2000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    wl = workload(args.workload, args.seed)
+    n, dim = wl["x"].shape
+    log(f"workload {wl['desc']} precision={args.precision}")
+    eng = make_engine(wl, args.precision, args.accumulate)
+    eng.step(args.warmup, "gjf")
+    sync_all(eng)
+    counts = eng.neighbor_counts()
+    kbar = float(counts.mean())
+    log(f"warmup {args.warmup} done, mean list length {kbar:.1f}")
+    # timed region: direct launches with HIP events around every step kernel
+    dt, ms, launches, rebuilds = timed_run(eng, args.steps, timing=True)
+    value = n * args.steps / dt
+    # the same number of steps replayed from captured hipGraph blocks (no events)
+    dtg, _, _, _ = timed_run(eng, args.steps, timing=False)
+    log(f"timed {args.steps} steps: {dt:.3f}s -> {value:.4e} particle-steps/s; rebuilds={rebuilds}; "
+        f"step kernel {ms / max(launches, 1) * 1e3:.1f} us/launch over {launches} launches")
+    # algorithmic bytes per particle-step (force + integrate), mixed / FP64
+    if args.precision == "mixed":
+        b_pp = 4 * kbar + dim * 4 + dim * 8 + 76 * dim
+    else:
+        b_pp = 4 * kbar + dim * 8 + dim * 8 + 72 * dim
+    avg_launch_s = ms / 1e3 / max(launches, 1)
+    achieved = n * b_pp / avg_launch_s / 1e9
+    out = {
+        "metric": "particle-steps/s on 1 MI355X, mixed vs FP64 GPU, at stated force error",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if args.precision == "mixed" else "f64",
+        "data": "synthetic (seeded lattice + Gaussian velocities)",
+        "config": {"workload": wl["desc"] + f" {args.precision}", "n_particles": n, "dim": dim,
+                   "precision": args.precision, "accumulate": args.accumulate, "seed": args.seed},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "dp_step_gjf (fused force + G-JF integrate)",
+                     "bytes_per_particle": b_pp, "mean_list_length": kbar,
+                     "avg_launch_us": avg_launch_s * 1e6},
+        "graph_replay_value": n * args.steps / dtg,
+        "rebuilds_in_timed_region": rebuilds,
+        "rebuild_interval": args.steps / max(rebuilds, 1),
+    }
+    x_now = eng.get_field("positions")
+    if not args.no_fp64 and args.precision == "mixed":
+        log("FP64 comparison run")
+        eng64 = make_engine(wl, "fp64")
+        eng64.step(args.warmup, "gjf")
+        k64 = max(args.steps // 4, 50)
+        dt64, ms64, l64, rb64 = timed_run(eng64, k64, timing=True)
+        v64 = n * k64 / dt64
+        out["fp64_value"] = v64
+        out["mixed_over_fp64"] = value / v64
+        out["fp64_step_kernel_us"] = ms64 / max(l64, 1) * 1e3
+        log(f"fp64: {v64:.4e} particle-steps/s (ratio {value / v64:.2f})")
+        # errors of the mixed kernels on the mixed run's configuration
+        eng64.set_field("positions", x_now)
+        r64 = eng64.compute(forces=True, energy=True, virial=True)
+        rmx = eng.compute(forces=True, energy=True, virial=True)
+        out["mixed_vs_fp64_gpu"] = force_errors(rmx["forces"], r64["forces"])
+        out["mixed_vs_fp64_gpu"]["energy_rel"] = float(abs(rmx["energy"][0] - r64["energy"][0]) / abs(r64["energy"][0]))
+        out["mixed_vs_fp64_gpu"]["virial_rel"] = float(abs(rmx["virial"][0] - r64["virial"][0]) / abs(r64["virial"][0]))
+        eng64.close()
+    if not args.no_nve:
+        log("NVE drift segments")
+        nve = {}
+        for prec in (("mixed", "fp64") if args.precision == "mixed" and not args.no_fp64 else (args.precision,)):
+            e2 = make_engine(wl, prec)
+            e2.set_field("positions", x_now)
+            e2.set_field("velocities", eng.get_field("velocities"))
+            e2.build_neighbor_list()
+            e2.set_integrator(wl["dt"], wl["T"], np.inf)
+            slope, sd = nve_drift(e2, args.nve_steps, 100)
+            nve[prec] = {"drift_per_particle_per_time": slope / wl["dt"], "energy_std_per_particle": sd}
+            e2.close()
+        out["nve"] = {"steps": args.nve_steps, **nve}
+    if not args.no_cpu:
+        log("CPU baseline (C oracle)")
+        cb, Fcpu = cpu_baseline(wl, x_now, out["rebuild_interval"])
+        out["cpu_baseline"] = cb
+        Fg = eng.compute(forces=True)["forces"]
+        out["mixed_vs_c_oracle" if args.precision == "mixed" else "fp64_vs_c_oracle"] = force_errors(Fg, Fcpu)
+        log(f"cpu: {cb['value']:.3e} particle-steps/s on {cb['cores']} threads")
+    else:
+        out["cpu_baseline"] = None
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -547,29 +547,39 @@ int pre_step_checks(dp_ctx* ctx) {
   return DP_OK;
 }
 
-int launch_one_step(dp_ctx* ctx, int integ, const double* xi_dev) {
+int get_event_pair(dp_ctx* ctx, std::pair<hipEvent_t, hipEvent_t>& ev) {
+  if (!ctx->ev_free.empty()) {
+    ev = ctx->ev_free.back();
+    ctx->ev_free.pop_back();
+    return DP_OK;
+  }
+  HIPCHK(hipEventCreate(&ev.first));
+  HIPCHK(hipEventCreate(&ev.second));
+  return DP_OK;
+}
+
+// One step: control, gated rebuild, fused force+integrate kernel.  With
+// `ev`, the step kernel is bracketed by the two events (also inside a
+// captured graph, where they become event-record nodes).
+int launch_one_step(dp_ctx* ctx, int integ, const double* xi_dev, const std::pair<hipEvent_t, hipEvent_t>* ev) {
   CHK(launch_prologue(ctx, 1));
   CHK(launch_rebuild(ctx));
   DpPairArgs a = pair_args(ctx, ctx->cur);
   a.xi = xi_dev;
   hipFunction_t f = (integ == DP_INTEGRATOR_GJF) ? ctx->f_gjf : ctx->f_lf;
-  if (ctx->timing) {
-    std::pair<hipEvent_t, hipEvent_t> ev;
-    if (!ctx->ev_free.empty()) {
-      ev = ctx->ev_free.back();
-      ctx->ev_free.pop_back();
-    } else {
-      HIPCHK(hipEventCreate(&ev.first));
-      HIPCHK(hipEventCreate(&ev.second));
-    }
-    HIPCHK(hipEventRecord(ev.first, ctx->st));
-    CHK(launch_pair(ctx, f, a));
-    HIPCHK(hipEventRecord(ev.second, ctx->st));
-    ctx->ev_pending.push_back(ev);
-  } else {
-    CHK(launch_pair(ctx, f, a));
-  }
+  if (ev) HIPCHK(hipEventRecord(ev->first, ctx->st));
+  CHK(launch_pair(ctx, f, a));
+  if (ev) HIPCHK(hipEventRecord(ev->second, ctx->st));
   ctx->cur = 1 - ctx->cur;
+  return DP_OK;
+}
+
+int launch_one_step_timed(dp_ctx* ctx, int integ, const double* xi_dev) {
+  if (!ctx->timing) return launch_one_step(ctx, integ, xi_dev, nullptr);
+  std::pair<hipEvent_t, hipEvent_t> ev;
+  CHK(get_event_pair(ctx, ev));
+  CHK(launch_one_step(ctx, integ, xi_dev, &ev));
+  ctx->ev_pending.push_back(ev);
   return DP_OK;
 }
 
@@ -593,7 +603,8 @@ int capture_graph(dp_ctx* ctx, int integ) {
   const int saved_cur = ctx->cur;
   HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
   int rc = DP_OK;
-  for (int s = 0; s < ctx->graph_block && rc == DP_OK; ++s) rc = launch_one_step(ctx, integ, nullptr);
+  for (int s = 0; s < ctx->graph_block && rc == DP_OK; ++s)
+    rc = launch_one_step(ctx, integ, nullptr, nullptr);
   hipError_t e = hipStreamEndCapture(ctx->st, &g);
   ctx->cur = saved_cur;
   if (rc != DP_OK) {
@@ -1066,25 +1077,28 @@ DP_API int dp_step(dp_ctx* ctx, int64_t nsteps, int integrator, const double* xi
   CHK(pre_step_checks(ctx));
   if (xi) {
     HIPCHK(hipMemcpyAsync(ctx->xi, xi, (size_t)ctx->N * ctx->D * 8, hipMemcpyDefault, ctx->st));
-    CHK(launch_one_step(ctx, integrator, ctx->xi));
+    CHK(launch_one_step_timed(ctx, integrator, ctx->xi));
     CHK(resolve_timing(ctx));
     return check_ctl(ctx);
   }
   int64_t left = nsteps;
+  // timing brackets every step kernel with events; events recorded inside a
+  // captured graph cannot be timed, so timed runs launch directly
   const bool use_graph = ctx->graph_block > 0 && !ctx->timing && nsteps >= 2 * (int64_t)ctx->graph_block;
   if (use_graph) {
     if (ctx->cur != 0) {
-      CHK(launch_one_step(ctx, integrator, nullptr));
+      CHK(launch_one_step_timed(ctx, integrator, nullptr));
       --left;
     }
-    if (ctx->graph_dirty || !ctx->gexec || ctx->gexec_integ != integrator) CHK(capture_graph(ctx, integrator));
+    if (ctx->graph_dirty || !ctx->gexec || ctx->gexec_integ != integrator)
+      CHK(capture_graph(ctx, integrator));
     while (left >= ctx->graph_block) {
       HIPCHK(hipGraphLaunch(ctx->gexec, ctx->st));
       left -= ctx->graph_block;
     }
   }
   while (left > 0) {
-    CHK(launch_one_step(ctx, integrator, nullptr));
+    CHK(launch_one_step_timed(ctx, integrator, nullptr));
     --left;
     if (ctx->timing && ctx->ev_pending.size() >= 256) CHK(resolve_timing(ctx));
   }

@@ -144,8 +144,9 @@ int dp_get_stats(dp_ctx* ctx, int64_t* nl_updates, int64_t* steps, int32_t* max_
 int dp_set_graph_block(dp_ctx* ctx, int steps);
 
 /* Kernel timing: when enabled, each fused step-kernel launch is bracketed by
- * HIP events on the context stream (graphs are disabled meanwhile);
- * dp_get_kernel_time returns the summed milliseconds and launch count. */
+ * HIP events on the context stream (steps are launched directly, not from a
+ * captured graph, while timing is on); dp_get_kernel_time returns the summed
+ * milliseconds and the number of launches timed since timing was enabled. */
 int dp_set_kernel_timing(dp_ctx* ctx, int enable);
 int dp_get_kernel_time(dp_ctx* ctx, double* total_ms, int64_t* launches);
 
