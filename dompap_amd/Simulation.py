@@ -323,25 +323,21 @@ class Simulation:
             eng.set_integrator(float(self.time_step), float(self.temperature_target),
                                float(self.temperature_damping_time))
             self.__dict__["_integ_dirty"] = False
+        bad = self.__dict__.setdefault("_bad_fields", {})
         for f in ("positions", "velocities", "betas", "image_positions", "masses"):
             if f in self._dirty:
-                v = self._h[f]
-                if f == "masses":
-                    v = np.asarray(v, dtype=np.float64).reshape(-1)
-                    if v.shape[0] != n:
-                        # only the integrator reads masses; like the reference,
-                        # a stale array is an error only when stepping
-                        self.__dict__["_masses_ok"] = False
-                        continue
-                    self.__dict__["_masses_ok"] = True
-                elif f == "image_positions":
-                    v = np.asarray(v).astype(np.int32)
-                else:
-                    v = np.asarray(v, dtype=np.float64)
-                if f != "masses" and v.shape != (n, dim):
-                    if f in ("velocities", "betas") and v.size != n * dim:
-                        raise ValueError(f"{f} has shape {v.shape}, expected {(n, dim)}")
-                eng.set_field(f, v.reshape(n, dim) if f != "masses" else v)
+                v = np.asarray(self._h[f])
+                want = (n,) if f == "masses" else (n, dim)
+                if v.size != int(np.prod(want)):
+                    if f == "positions":
+                        raise ValueError(f"positions has shape {v.shape}, expected {want}")
+                    # only the integrator reads these; as in the reference, a
+                    # stale array of another shape is an error only when stepping
+                    bad[f] = v.shape
+                    continue
+                bad.pop(f, None)
+                v = v.astype(np.int32 if f == "image_positions" else np.float64).reshape(want)
+                eng.set_field(f, v)
                 self._dirty.discard(f)
         if self._restore_xref is not None:
             xr = np.asarray(self._restore_xref, dtype=np.float64)
@@ -621,9 +617,11 @@ class Simulation:
         self._dirty.update(("positions", "image_positions"))
 
     def _check_masses(self):
-        if not self.__dict__.get("_masses_ok", True):
-            raise ValueError(f"Expected shape of masses: {(self.number_of_particles(), 1)}, "
-                             f"got: {np.shape(self._h['masses'])}")
+        bad = self.__dict__.get("_bad_fields", {})
+        if bad:
+            f, shp = next(iter(bad.items()))
+            raise ValueError(f"{f} has shape {shp}, expected "
+                             f"{(self.number_of_particles(), 1 if f == 'masses' else self.get_dimensions_of_space())}")
 
     def step(self, noise=None) -> None:
         """Simulation.py:681-704: skin check, (rebuild), forces, G-JF step,

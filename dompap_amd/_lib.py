@@ -61,7 +61,7 @@ SIGNATURES = {
     "dp_update_neighbor_list": (_c_int, [_vp, ctypes.POINTER(_c_int)]),
     "dp_compute": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
     "dp_step": (_c_int, [_vp, _c_i64, _c_int, _vp]),
-    "dp_integrate": (_c_int, [_vp, _vp, _c_int, _vp]),
+    "dp_integrate": (_c_int, [_vp, _vp, _c_int, _vp, _c_int]),
     "dp_set_seed": (_c_int, [_vp, _c_u64, _c_u64]),
     "dp_get_neighbor_list": (_c_int, [_vp, _vp]),
     "dp_get_neighbor_counts": (_c_int, [_vp, _vp]),

@@ -104,6 +104,8 @@ struct DpPairArgs {
   u64 seed;
   DpCtl* ctl;
   u64* dispmax;           // [R] bit patterns of max displacement^2
+  i32 wrap;               // 1: wrap positions into the box after the update
+  i32 pad_;
 };
 
 // Neighbour-list entry: slot index << (2*D) | 2-bit stencil code per dim

@@ -112,7 +112,7 @@ int compile_cached(const std::string& src, std::vector<char>& code, std::string&
     return DP_ERR_RTC;
   }
   // packed-FP32 instructions are disabled for all HIP code of this project
-  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=fast",
+  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=fast-honor-pragmas",
                         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
   hiprtcResult rr = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
   size_t ls = 0;
@@ -406,6 +406,7 @@ DpPairArgs pair_args(dp_ctx* ctx, int cur) {
   a.seed = ctx->seed;
   a.ctl = ctx->ctl_d;
   a.dispmax = ctx->dispmax_d;
+  a.wrap = 1;
   return a;
 }
 
@@ -1106,7 +1107,7 @@ DP_API int dp_step(dp_ctx* ctx, int64_t nsteps, int integrator, const double* xi
   return check_ctl(ctx);
 }
 
-DP_API int dp_integrate(dp_ctx* ctx, const double* forces, int integrator, const double* xi) {
+DP_API int dp_integrate(dp_ctx* ctx, const double* forces, int integrator, const double* xi, int wrap) {
   if (!ctx || !forces) return DP_ERR_ARG;
   (void)hipSetDevice(ctx->dev);
   if (integrator != DP_INTEGRATOR_GJF && integrator != DP_INTEGRATOR_LEAPFROG)
@@ -1121,6 +1122,7 @@ DP_API int dp_integrate(dp_ctx* ctx, const double* forces, int integrator, const
   DpPairArgs a = pair_args(ctx, ctx->cur);
   a.F = ctx->F;
   a.xi = xi ? ctx->xi : nullptr;
+  a.wrap = wrap ? 1 : 0;
   CHK(launch_pair(ctx, integrator == DP_INTEGRATOR_GJF ? ctx->f_int_gjf : ctx->f_int_lf, a));
   ctx->cur = 1 - ctx->cur;
   ctx->pos_dirty = true;  // the list's skin check must see the new positions

@@ -287,9 +287,11 @@ __device__ __forceinline__ void dp_step_body(const DpPairArgs& a) {
     // positions.py:90-98 wrap, then neighbor_list.py:183-201 displacement
 #pragma unroll
     for (int d = 0; d < DP_D; ++d) {
-      const double sh = floor(xn[d] / R.L[d]);
-      a.img[ii * DP_D + d] += (i32)sh;
-      xn[d] = xn[d] - sh * R.L[d];
+      if (a.wrap) {
+        const double sh = floor(xn[d] / R.L[d]);
+        a.img[ii * DP_D + d] += (i32)sh;
+        xn[d] = xn[d] - sh * R.L[d];
+      }
       a.x_out[ii * DP_D + d] = xn[d];
       double t = dp_sub(xn[d], a.xref[ii * DP_D + d]);
       t = dp_minimage_gt(t, R.L[d], R.halfL[d]);

@@ -194,14 +194,14 @@ class Engine:
             xp = _ptr(xi)
         self._chk(self.lib.dp_step(self.ctx, int(nsteps), integ, xp))
 
-    def integrate(self, forces, integrator: str = "gjf", xi=None) -> None:
+    def integrate(self, forces, integrator: str = "gjf", xi=None, wrap: bool = True) -> None:
         integ = _lib.DP_INTEGRATOR_GJF if integrator == "gjf" else _lib.DP_INTEGRATOR_LEAPFROG
         f = _as_f64(forces, (self.n, self.dim))
         xp = None
         if xi is not None:
             xi = _as_f64(xi, (self.n, self.dim))
             xp = _ptr(xi)
-        self._chk(self.lib.dp_integrate(self.ctx, _ptr(f), integ, xp))
+        self._chk(self.lib.dp_integrate(self.ctx, _ptr(f), integ, xp, int(bool(wrap))))
 
     def sync(self) -> None:
         self._chk(self.lib.dp_sync(self.ctx))

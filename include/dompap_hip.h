@@ -124,9 +124,10 @@ int dp_step(dp_ctx* ctx, int64_t nsteps, int integrator, const double* xi);
 
 /* One integrator update with caller-supplied forces[N][D] (no pair loop):
  * make_one_step (integrator.py:17-40) / make_one_step_leap_frog
- * (integrator.py:5-14) followed by the wrap of positions.py:90-98.  xi as in
+ * (integrator.py:5-14), followed by the wrap of positions.py:90-98 when
+ * wrap != 0 (the reference's free functions do not wrap).  xi as in
  * dp_step. */
-int dp_integrate(dp_ctx* ctx, const double* forces, int integrator, const double* xi);
+int dp_integrate(dp_ctx* ctx, const double* forces, int integrator, const double* xi, int wrap);
 
 /* RNG key and step counter. */
 int dp_set_seed(dp_ctx* ctx, uint64_t seed, uint64_t step_counter);
