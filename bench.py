@@ -70,14 +70,14 @@ def workload(name, seed):
                 dt=0.005, T=T, tau=10.0, K=256, seed=seed)
 
 
-def make_engine(wl, precision, accumulate="fp64"):
+def make_engine(wl, precision, pair_precision="fp64"):
     from dompap_amd.engine import Engine
     from dompap_amd.potential import device_potential
     n, dim = wl["x"].shape
     eng = Engine(dim, n, 1, precision)
     eng.set_box(wl["box"])
     dp = device_potential(wl["pot"], wl["r_cut"])
-    eng.load_potential(dp.code, dp.r_cut, dp.u_rc, [[1.0]], [[1.0]], accumulate)
+    eng.load_potential(dp.code, dp.r_cut, dp.u_rc, [[1.0]], [[1.0]], pair_precision)
     eng.set_neighbor_params(1.0 * wl["r_cut"] + wl["skin"], wl["skin"], wl["K"])
     eng.set_integrator(wl["dt"], wl["T"], wl["tau"])
     eng.set_seed(wl["seed"], 0)
@@ -185,7 +185,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", default="lj-1m")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
-    ap.add_argument("--accumulate", default="fp64", choices=["fp64", "kahan"])
+    ap.add_argument("--pair-precision", default="fp64", choices=["fp64", "fp32"],
+                    help="mixed mode: precision of the in-range pair terms")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -203,7 +204,7 @@ def main():
     wl = workload(args.workload, args.seed)
     n, dim = wl["x"].shape
     log(f"workload {wl['desc']} precision={args.precision}")
-    eng = make_engine(wl, args.precision, args.accumulate)
+    eng = make_engine(wl, args.precision, args.pair_precision)
     eng.step(args.warmup, "gjf")
     sync_all(eng)
     counts = eng.neighbor_counts()
@@ -237,7 +238,7 @@ def main():
         "dtype": "f32+f64" if args.precision == "mixed" else "f64",
         "data": "synthetic (seeded lattice + Gaussian velocities)",
         "config": {"workload": wl["desc"] + f" {args.precision}", "n_particles": n, "dim": dim,
-                   "precision": args.precision, "accumulate": args.accumulate, "seed": args.seed},
+                   "precision": args.precision, "pair_precision": args.pair_precision, "seed": args.seed},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "dp_step_gjf (fused force + G-JF integrate)",

@@ -150,7 +150,7 @@ class Simulation:
                  number_of_steps=0, time_step=np.float64(0.01), temperature_target=np.float64(1.0),
                  temperature_damping_time=np.float64(0.1), pair_potential=None, pair_potential_str=None,
                  pair_force=None, pair_curvature=None, epsilon_func=None, sigma_func=None,
-                 precision="fp64", device=0, seed=None, accumulate="fp64"):
+                 precision="fp64", device=0, seed=None, pair_precision="fp64"):
         d = self.__dict__
         d["_engine"] = None
         d["_h"] = {}
@@ -158,7 +158,7 @@ class Simulation:
         d["_dirty"] = set()
         d["_tok"] = {}
         d["_precision"] = precision
-        d["_accumulate"] = accumulate
+        d["_pair_precision"] = pair_precision
         d["_device_index"] = int(device)
         d["seed"] = int(np.random.randint(0, 2**62)) if seed is None else int(seed)
         d["_params_dirty"] = True
@@ -364,7 +364,7 @@ class Simulation:
         tabs = lower_parameters(self.sigma_func, self.epsilon_func, types, self._sigma_const, self._epsilon_const)
         self.__dict__["_tables"] = tabs
         dp = device_potential(self.pair_potential_str, float(self.pair_potential_r_cut))
-        eng.load_potential(dp.code, dp.r_cut, dp.u_rc, tabs.sigma, tabs.epsilon, self._accumulate)
+        eng.load_potential(dp.code, dp.r_cut, dp.u_rc, tabs.sigma, tabs.epsilon, self._pair_precision)
         eng.set_field("types", tabs.types)
         self._dirty.discard("particle_types")
         self.__dict__["_pot_dirty"] = False

@@ -1,5 +1,5 @@
 // dp_device.h -- device-side data layout and helpers shared by the hipcc-built
-// pipeline kernels (dp_kernels.hip) and the hiprtc-built pair kernels
+// pipeline kernels (dp_kernels.h) and the hiprtc-built pair kernels
 // (dp_pair.hip).  Plain C structs only, so that the host library, hipcc and
 // hiprtc all see the same layout.
 #pragma once
@@ -9,6 +9,7 @@ typedef unsigned int u32;
 typedef int i32;
 typedef unsigned long long u64;
 typedef long long i64;
+typedef unsigned short u16;
 #else
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -16,30 +17,41 @@ typedef uint32_t u32;
 typedef int32_t i32;
 typedef uint64_t u64;
 typedef int64_t i64;
+typedef uint16_t u16;
 #endif
 
-#define DP_DMAX 8          // largest spatial dimension supported
+#define DP_DMAX 4          // largest spatial dimension supported (stencil rows fit one wavefront)
 #define DP_MAXTYPES 8      // largest number of particle types (sigma/eps tables)
+#define DP_HALO_MAX 512    // halo cells per tile
+#define DP_HOME_MAX 128    // home cells per tile
+#define DP_TILE_THREADS 256
 
 // One independent system ("replica").  Particles of replica r occupy slots
-// [p0, p0+n) of every per-particle array; its cells occupy [c0, c0+ncell).
+// [p0, p0+n) of every per-particle array; its cells occupy [c0, c0+ncell);
+// its tiles (groups of home cells handled by one workgroup) [t0, t0+ntiles).
+// Cells and tiles are linearised with the stencil's fastest digit fastest:
+// dimension 0 fastest when `order` is 0, dimension D-1 fastest when it is 1.
 struct DpRep {
   i64 p0, n;            // particle slot range
   i64 c0, ncell;        // global cell range
+  i64 t0, ntiles;       // global tile range
   i32 nc[DP_DMAX];      // cells per dimension (>= 3, or 1 = collapsed)
+  i32 T[DP_DMAX];       // home cells per tile per dimension
+  i32 nt[DP_DMAX];      // tiles per dimension
+  i32 Th[DP_DMAX];      // halo box extent per dimension (T + 2, or 1 when collapsed)
+  i32 Wq[DP_DMAX];      // cell width in fixed-point units, rint(w * 2^qbits)
+  i32 Lq[DP_DMAX];      // box length in fixed-point units (collapsed dimensions)
   double L[DP_DMAX];    // box lengths
   double halfL[DP_DMAX];// L/2 (reference: box_vectors[d] / 2)
   double w[DP_DMAX];    // cell width L/nc
   double inv_w[DP_DMAX];// nc/L
-  float wf_hi[DP_DMAX]; // w as a float pair (hi + lo) for mixed precision
-  float wf_lo[DP_DMAX];
-  float Lf[DP_DMAX];    // box lengths as floats (collapsed dims only)
-  float invLf[DP_DMAX];
   double nl_thresh;     // s < nl_thresh  <=>  sqrt(s) < cutoff (exact, FP64)
   double skin_half_sq;  // (0.5 * skin)^2
-  double dt, T, tau;    // integrator parameters
+  double dt, T_, tau;   // integrator parameters
+  i32 nhalo;            // prod(Th)
+  i32 nhome;            // prod(T)
   i32 collapsed_mask;   // bit d set if nc[d] == 1
-  i32 pad_;
+  i32 order;            // 0: dimension 0 fastest, 1: dimension D-1 fastest
 };
 
 // Device control block (one per context).
@@ -51,7 +63,9 @@ struct DpCtl {
   i32 overflow;         // some row needed more than K entries
   i32 max_count;        // largest row length seen at the last emit
   i32 nonfinite;        // a non-finite position was seen while binning
-  i32 pad_;
+  i32 cell_overflow;    // a cell held more particles than the entry format allows
+  i32 max_cell;         // largest cell occupancy at the last rebuild
+  i32 max_halo;         // largest staged halo (records) at the last rebuild
 };
 
 // Pair-interaction tables (per context; indexed [ti * ntypes + tj]).
@@ -59,32 +73,37 @@ struct DpPairTab {
   double sigma[DP_MAXTYPES * DP_MAXTYPES];
   double eps[DP_MAXTYPES * DP_MAXTYPES];
   double cut_thresh[DP_MAXTYPES * DP_MAXTYPES];   // s > t  <=>  sqrt(s)/sigma > r_cut
-  double inv_sigma[DP_MAXTYPES * DP_MAXTYPES];
-  double inv_sigma2[DP_MAXTYPES * DP_MAXTYPES];
-  float sigma_f[DP_MAXTYPES * DP_MAXTYPES];
+  double inv_sigma2[DP_MAXTYPES * DP_MAXTYPES];   // 1/sigma^2
+  double eps_over_sigma[DP_MAXTYPES * DP_MAXTYPES];
+  float guard_lo_q[DP_MAXTYPES * DP_MAXTYPES];    // fixed-point r^2 below: certainly inside
+  float guard_hi_q[DP_MAXTYPES * DP_MAXTYPES];    // fixed-point r^2 at or above: certainly outside
+  float inv_sigma2_qf[DP_MAXTYPES * DP_MAXTYPES]; // 2^-2q / sigma^2 (FP32 pair path)
+  float eps_over_sigma_qf[DP_MAXTYPES * DP_MAXTYPES];  // eps / sigma * 2^-q (FP32 pair path)
   float eps_f[DP_MAXTYPES * DP_MAXTYPES];
-  float eps_over_sigma_f[DP_MAXTYPES * DP_MAXTYPES];
-  float inv_sigma_f[DP_MAXTYPES * DP_MAXTYPES];
-  float inv_sigma2_f[DP_MAXTYPES * DP_MAXTYPES];
-  float thresh_f[DP_MAXTYPES * DP_MAXTYPES];      // r^2 cut in FP32 (centre of guard band)
-  float guard_lo_f[DP_MAXTYPES * DP_MAXTYPES];    // below: certainly inside
-  float guard_hi_f[DP_MAXTYPES * DP_MAXTYPES];    // above: certainly outside
+  float sigma_f[DP_MAXTYPES * DP_MAXTYPES];
   double r_cut;
   double u_rc;          // u(r_cut) used for the energy shift
+  double qscale;        // 2^-qbits
+  double qscale2;       // 2^-2qbits
+  double qinv;          // 2^qbits
 };
 
 // Arguments of the hiprtc-built pair / step kernels (passed by value).
 struct DpPairArgs {
   // inputs (current buffers)
   const double* x;        // [N*D] FP64 positions, slot order
-  const float* xs;        // [N*RS] mixed records (cell-relative FP32 + type)
-  const u32* nl;          // [K*Ns] neighbour list, column major
+  const i32* xq;          // [N*RS] mixed records: fixed-point cell-relative coordinates + type
+  const void* nl;         // [N*K] neighbour entries (u16 or u32), row major
   const i32* nlcnt;       // [N]
   const i32* type;        // [N]
-  const i32* rep;         // [N]
+  const i32* cstart;      // [C+1] first slot of every cell
   const DpRep* reps;      // [R]
   const DpPairTab* tab;
-  i64 N, Ns;
+  i64 N;
+  i64 ntiles;             // total tiles (grid size)
+  i32 K, R;
+  i32 cap;                // records that fit in the workgroup's LDS
+  i32 pad0_;
   // observable outputs (nullable)
   double* F;              // [N*D]
   double* E;              // [N]
@@ -92,7 +111,7 @@ struct DpPairArgs {
   double* LAP;            // [N]
   // integrator state (fused step kernels)
   double* x_out;          // [N*D]
-  float* xs_out;          // [N*RS]
+  i32* xq_out;            // [N*RS]
   double* v;              // [N*D]
   double* beta;           // [N*D]
   i32* img;               // [N*D]
@@ -100,17 +119,14 @@ struct DpPairArgs {
   const double* xref;     // [N*D]
   const i32* cellpack;    // [N]
   const i32* orig;        // [N]
+  const i32* rep;         // [N]
   const double* xi;       // injected normals [N*D] in original order, or null
   u64 seed;
   DpCtl* ctl;
   u64* dispmax;           // [R] bit patterns of max displacement^2
   i32 wrap;               // 1: wrap positions into the box after the update
-  i32 pad_;
+  i32 pad1_;
 };
-
-// Neighbour-list entry: slot index << (2*D) | 2-bit stencil code per dim
-// (code = s + 1 with s in {-1,0,1}; j sits in cell c_i + s).
-#define DP_NL_SHIFT(D) (2 * (D))
 
 #ifndef DP_NO_DEVICE_HELPERS
 
@@ -145,16 +161,13 @@ __device__ __forceinline__ DpU4 dp_philox4x32_10(DpU4 c, u32 k0, u32 k1) {
   return c;
 }
 
-// Two standard normals from 128 random bits (Box-Muller on two 64-bit
-// uniforms in (0,1)).
+// Two standard normals from 128 random bits (Box-Muller on two 53-bit
+// uniforms strictly inside (0,1)).
 __device__ __forceinline__ void dp_box_muller(DpU4 b, double& z0, double& z1) {
-  const double two_m64 = 5.421010862427522170037e-20;  // 2^-64
   u64 a = ((u64)b.x << 32) | b.y;
   u64 c = ((u64)b.z << 32) | b.w;
-  // (a >> 11) gives 53 bits; +0.5 keeps u strictly inside (0,1)
   double u1 = ((double)(a >> 11) + 0.5) * 1.1102230246251565404e-16;  // 2^-53
   double u2 = ((double)(c >> 11) + 0.5) * 1.1102230246251565404e-16;
-  (void)two_m64;
   double rad = sqrt(-2.0 * log(u1));
   double th = 6.283185307179586476925 * u2;
   double s, co;
@@ -194,6 +207,178 @@ __device__ __forceinline__ double dp_minimage_lt(double d, double L, double half
   if (d < -halfL) return dp_add(d, L);
   if (d > halfL) return dp_sub(d, L);
   return d;
+}
+
+// ------------------------------------------------------------------------
+// Tile geometry.  A workgroup handles the home cells of one tile; the halo
+// box (home cells plus one layer, periodic) is staged in LDS.  Halo cells are
+// linearised in the stencil's digit order, so that the cells of a stencil
+// row along the fastest digit are consecutive.
+// ------------------------------------------------------------------------
+template <int D> struct DpPow3 { static constexpr int value = 3 * DpPow3<D - 1>::value; };
+template <> struct DpPow3<0> { static constexpr int value = 1; };
+
+template <int D>
+struct DpTileSmem {
+  i32 hstart[DP_HALO_MAX];     // first slot of each halo cell
+  i32 hbase[DP_HALO_MAX + 1];  // staged-record offset of each halo cell (prefix)
+  i32 home_u[DP_HOME_MAX];     // halo index of each home cell (-1: outside the box)
+  i32 home_p[DP_HOME_MAX + 1]; // home-particle prefix over home cells
+  i32 st_off[DpPow3<D>::value];// halo-index offset of each stencil entry
+  i32 nrec;                    // staged records in the halo
+  i32 nhomep;                  // home particles
+  i32 rep;                     // replica of the tile
+  i32 fits;                    // 1 if the halo fits in the LDS record area
+};
+
+// Replica owning global tile t.
+__device__ __forceinline__ int dp_tile_replica(const DpRep* reps, int R, i64 t) {
+  int lo = 0, hi = R - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (reps[mid].t0 <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Digits of a linear index in the replica's digit order.
+template <int D>
+__device__ __forceinline__ void dp_unlinear(i64 lin, const i32* ext, int order, int out[D]) {
+  if (order == 0) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) { out[d] = (int)(lin % ext[d]); lin /= ext[d]; }
+  } else {
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) { out[d] = (int)(lin % ext[d]); lin /= ext[d]; }
+  }
+}
+
+template <int D>
+__device__ __forceinline__ i64 dp_linear(const int* c, const i32* ext, int order) {
+  i64 lin = 0, stride = 1;
+  if (order == 0) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) { lin += (i64)c[d] * stride; stride *= ext[d]; }
+  } else {
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) { lin += (i64)c[d] * stride; stride *= ext[d]; }
+  }
+  return lin;
+}
+
+// Stencil digits s_d in {-1, 0, 1} of stencil index st.
+template <int D>
+__device__ __forceinline__ void dp_stencil_digits(int st, int order, int s[D]) {
+  if (order == 0) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) { s[d] = st % 3 - 1; st /= 3; }
+  } else {
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) { s[d] = st % 3 - 1; st /= 3; }
+  }
+}
+
+// Fill the tile tables for global tile t (every thread of the workgroup calls
+// this; it ends with a barrier).  `cap` is the LDS record capacity.
+template <int D>
+__device__ void dp_tile_setup(DpTileSmem<D>& S, const DpRep* reps, int R, const i32* cstart, i64 t, int cap) {
+  const int tid = threadIdx.x;
+  const int r = dp_tile_replica(reps, R, t);
+  const DpRep& P = reps[r];
+  int tc[D];
+  dp_unlinear<D>(t - P.t0, P.nt, P.order, tc);
+  const int nh = P.nhalo;
+  for (int u = tid; u < nh; u += blockDim.x) {
+    int hc[D];
+    dp_unlinear<D>(u, P.Th, P.order, hc);
+    int cc[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      if (P.nc[d] == 1) {
+        cc[d] = 0;
+      } else {
+        int c = tc[d] * P.T[d] + hc[d] - 1;
+        c %= P.nc[d];
+        if (c < 0) c += P.nc[d];
+        cc[d] = c;
+      }
+    }
+    const i64 cell = P.c0 + dp_linear<D>(cc, P.nc, P.order);
+    const i32 s0 = cstart[cell];
+    S.hstart[u] = s0;
+    S.hbase[u + 1] = cstart[cell + 1] - s0;
+  }
+  const int nhm = P.nhome;
+  for (int m = tid; m < nhm; m += blockDim.x) {
+    int mc[D];
+    dp_unlinear<D>(m, P.T, P.order, mc);
+    int hc[D];
+    bool valid = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const bool coll = P.nc[d] == 1;
+      hc[d] = coll ? 0 : mc[d] + 1;
+      if (!coll && tc[d] * P.T[d] + mc[d] >= P.nc[d]) valid = false;
+    }
+    S.home_u[m] = valid ? (int)dp_linear<D>(hc, P.Th, P.order) : -1;
+  }
+  constexpr int n3 = DpPow3<D>::value;
+  for (int st = tid; st < n3; st += blockDim.x) {
+    int sd[D];
+    dp_stencil_digits<D>(st, P.order, sd);
+    int off = 0, stride = 1;
+    if (P.order == 0) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if (P.nc[d] != 1) off += sd[d] * stride;
+        stride *= P.Th[d];
+      }
+    } else {
+#pragma unroll
+      for (int d = D - 1; d >= 0; --d) {
+        if (P.nc[d] != 1) off += sd[d] * stride;
+        stride *= P.Th[d];
+      }
+    }
+    S.st_off[st] = off;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    S.hbase[0] = 0;
+    for (int u = 0; u < nh; ++u) S.hbase[u + 1] += S.hbase[u];
+    S.nrec = S.hbase[nh];
+    S.home_p[0] = 0;
+    for (int m = 0; m < nhm; ++m) {
+      const int u = S.home_u[m];
+      S.home_p[m + 1] = S.home_p[m] + (u >= 0 ? S.hbase[u + 1] - S.hbase[u] : 0);
+    }
+    S.nhomep = S.home_p[nhm];
+    S.rep = r;
+    S.fits = S.nrec <= cap ? 1 : 0;
+  }
+  __syncthreads();
+}
+
+// Halo cell holding staged record idx (binary search over hbase).
+template <int D>
+__device__ __forceinline__ int dp_halo_of(const DpTileSmem<D>& S, int nh, int idx) {
+  int lo = 0, hi = nh - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (S.hbase[mid] <= idx) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Home cell holding home particle k (binary search over home_p).
+template <int D>
+__device__ __forceinline__ int dp_home_of(const DpTileSmem<D>& S, int nhm, int k) {
+  int lo = 0, hi = nhm - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (S.home_p[mid] <= k) lo = mid; else hi = mid - 1;
+  }
+  return lo;
 }
 
 #endif  // DP_NO_DEVICE_HELPERS

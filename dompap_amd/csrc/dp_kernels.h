@@ -43,7 +43,11 @@ __global__ void k_prologue(DpCtl* ctl, u64* dispmax, const DpRep* reps, int R, i
   const int need = old | ctl->force_rebuild;
   ctl->rebuild = need;
   ctl->force_rebuild = 0;
-  if (need) ctl->max_count = 0;
+  if (need) {
+    ctl->max_count = 0;
+    ctl->max_cell = 0;
+    ctl->max_halo = 0;
+  }
   // only skin-triggered rebuilds are counted here; requested ones are counted
   // by the caller that requested them
   if (old) {
@@ -96,21 +100,20 @@ __global__ void k_bin(const double* x, const i32* rep, const DpRep* reps, i64 N,
   if (!ctl->rebuild) return;
   DP_FOR(i, N) {
     const DpRep& R = reps[rep[i]];
-    i64 lin = 0, stride = 1;
+    int c[D];
     bool finite = true;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       double xd = x[i * D + d];
       if (!(xd - xd == 0.0)) { finite = false; xd = 0.0; }
       xd = xd - floor(xd / R.L[d]) * R.L[d];
-      int c = (int)floor(xd * R.inv_w[d]);
-      if (c < 0) c = 0;
-      if (c >= R.nc[d]) c = R.nc[d] - 1;
-      lin += (i64)c * stride;
-      stride *= R.nc[d];
+      int cd = (int)floor(xd * R.inv_w[d]);
+      if (cd < 0) cd = 0;
+      if (cd >= R.nc[d]) cd = R.nc[d] - 1;
+      c[d] = cd;
     }
     if (!finite) ctl->nonfinite = 1;
-    const i32 cid = (i32)(R.c0 + lin);
+    const i32 cid = (i32)(R.c0 + dp_linear<D>(c, R.nc, R.order));
     cell_of[i] = cid;
     rank[i] = atomicAdd(&count[cid], 1);
   }
@@ -246,14 +249,21 @@ __global__ void k_permute(const i32* perm, i64 N, const double* x, const double*
   }
 }
 
+// Fixed-point cell-relative coordinate of a position component: the offset
+// from the centre of the particle's cell (minimum image), scaled by 2^qbits.
+__device__ __forceinline__ i32 dp_quantise(double xd, double cen, double L, double halfL, double qinv) {
+  return (i32)rint(dp_minimage_gt(xd - cen, L, halfL) * qinv);
+}
+
 // Copy back from scratch, record the reference positions (Simulation.py:381
 // positions_neighbour_list = positions.copy()), pack cell coordinates, and
-// write the mixed-precision records (FP32 position relative to the cell
-// centre; type bits in the slot after the coordinates).
+// write the mixed-precision records (fixed-point offsets from the cell
+// centre; type in the slot after the coordinates).
 template <int D>
 __global__ void k_unpermute_store(i64 N, double* x, double* v, double* beta, double* mass, i32* img, i32* type,
                                   i32* orig, i32* cell_of, const char* scratch, double* xref, i32* cellpack,
-                                  float* xs, int rs, const i32* rep, const DpRep* reps, const DpCtl* ctl) {
+                                  i32* xq, int rs, double qinv, const i32* rep, const DpRep* reps,
+                                  const DpCtl* ctl) {
   if (!ctl->rebuild) return;
   const double* sx = (const double*)scratch;
   const double* sv = sx + N * D;
@@ -266,125 +276,177 @@ __global__ void k_unpermute_store(i64 N, double* x, double* v, double* beta, dou
   DP_FOR(q, N) {
     const DpRep& R = reps[rep[q]];
     const i32 cid = sc[q];
-    i64 lin = (i64)cid - R.c0;
+    int c[D];
+    dp_unlinear<D>((i64)cid - R.c0, R.nc, R.order, c);
     u32 pack = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const i64 c = lin % R.nc[d];
-      lin /= R.nc[d];
-      pack |= ((u32)c) << (DpPack<D>::BITS * d);
+      pack |= ((u32)c[d]) << (DpPack<D>::BITS * d);
       const double xd = sx[q * D + d];
       x[q * D + d] = xd;
       xref[q * D + d] = xd;
       v[q * D + d] = sv[q * D + d];
       beta[q * D + d] = sb[q * D + d];
       img[q * D + d] = si[q * D + d];
-      if (xs) {
-        const double cen = ((double)c + 0.5) * R.w[d];
-        xs[q * rs + d] = (float)dp_minimage_gt(xd - cen, R.L[d], R.halfL[d]);
-      }
+      if (xq) xq[q * rs + d] = dp_quantise(xd, ((double)c[d] + 0.5) * R.w[d], R.L[d], R.halfL[d], qinv);
     }
     mass[q] = sm[q];
     type[q] = st[q];
     orig[q] = so[q];
     cell_of[q] = cid;
     cellpack[q] = (i32)pack;
-    if (xs) {
-      if (D < rs) xs[q * rs + D] = __int_as_float(st[q]);
-      for (int k = D + 1; k < rs; ++k) xs[q * rs + k] = 0.f;
+    if (xq) {
+      if (D < rs) xq[q * rs + D] = st[q];
+      for (int k = D + 1; k < rs; ++k) xq[q * rs + k] = 0;
     }
   }
 }
 
 // Mixed records from the current positions (after host uploads).
 template <int D>
-__global__ void k_make_records(i64 N, const double* x, const i32* type, const i32* cellpack, float* xs, int rs,
-                               const i32* rep, const DpRep* reps) {
+__global__ void k_make_records(i64 N, const double* x, const i32* type, const i32* cellpack, i32* xq, int rs,
+                               double qinv, const i32* rep, const DpRep* reps) {
   DP_FOR(q, N) {
     const DpRep& R = reps[rep[q]];
     const u32 pack = (u32)cellpack[q];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const u32 c = (pack >> (DpPack<D>::BITS * d)) & DpPack<D>::MASK;
-      const double cen = ((double)c + 0.5) * R.w[d];
-      xs[q * rs + d] = (float)dp_minimage_gt(x[q * D + d] - cen, R.L[d], R.halfL[d]);
+      xq[q * rs + d] = dp_quantise(x[q * D + d], ((double)c + 0.5) * R.w[d], R.L[d], R.halfL[d], qinv);
     }
-    if (D < rs) xs[q * rs + D] = __int_as_float(type[q]);
-    for (int k = D + 1; k < rs; ++k) xs[q * rs + k] = 0.f;
+    if (D < rs) xq[q * rs + D] = type[q];
+    for (int k = D + 1; k < rs; ++k) xq[q * rs + k] = 0;
   }
 }
 
 // ---------------------------------------------------------------------------
-// Neighbour-list emit: for each particle, the stencil cells in the
-// reference's order (neighbor_list.py:150-157, x fastest, for order == 0;
-// neighbor_list.py:82-86, last dimension fastest, for order == 1), members in
-// ascending original id, kept iff the FP64 distance test of
+// Neighbour-list emit, one workgroup per tile.  The halo box's FP64 positions
+// are staged in LDS; each wavefront takes one home particle at a time and
+// sweeps its stencil rows (3 consecutive halo cells along the fastest digit)
+// 64 candidates at a time.  A candidate is kept iff the FP64 distance test of
 // neighbor_list.py:24-33 holds (evaluated as s < threshold, which is exactly
-// sqrt(s) < cutoff).  Entries are slot << 2D | stencil code.
+// sqrt(s) < cutoff); ballot compaction keeps the stencil order of
+// neighbor_list.py:150-157 (order 0) / :82-86 (order 1) and ascending
+// original id inside a cell.  Entries are (stencil index << ob) | offset of
+// the neighbour inside its cell.
 // ---------------------------------------------------------------------------
-template <int D>
-__global__ void __launch_bounds__(DP_BLOCK) k_nl_emit(const double* x, const i32* cellpack, const i32* cstart,
-                                                      const i32* rep, const DpRep* reps, i64 N, i64 Ns, int K,
-                                                      int order, u32* nl, i32* nlcnt, DpCtl* ctl) {
+template <int D, typename E>
+__global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile(const double* x, const i32* cstart,
+                                                               const DpRep* reps, int R, i64 ntiles, int K,
+                                                               int cap, int ob, E* nl, i32* nlcnt, DpCtl* ctl) {
   if (!ctl->rebuild) return;
-  DP_FOR(i, N) {
-    const DpRep& R = reps[rep[i]];
-    double xi[D];
-    int ci[D];
-    const u32 pack = (u32)cellpack[i];
+  constexpr int N3 = DpPow3<D>::value;
+  constexpr int NROW = N3 / 3;  // <= 27 for D <= 4: one row per lane
+  __shared__ DpTileSmem<D> S;
+  extern __shared__ double sx[];  // cap * D staged positions
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int nwaves = blockDim.x >> 6;
+  const u32 offmax = (1u << ob) - 1u;
+  for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    dp_tile_setup<D>(S, reps, R, cstart, t, cap);
+    const DpRep& P = reps[S.rep];
+    const int nh = P.nhalo;
+    const bool fits = S.fits != 0;
+    const int fastest = (P.order == 0) ? 0 : D - 1;
+    const int dx0 = (P.nc[fastest] == 1) ? 1 : 0;  // only s = 0 along a collapsed fastest dimension
+    if (fits) {
+      for (int idx = tid; idx < S.nrec; idx += blockDim.x) {
+        const int u = dp_halo_of<D>(S, nh, idx);
+        const i64 slot = S.hstart[u] + (idx - S.hbase[u]);
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      xi[d] = x[i * D + d];
-      ci[d] = (int)((pack >> (DpPack<D>::BITS * d)) & DpPack<D>::MASK);
-    }
-    int n3 = 1;
-#pragma unroll
-    for (int d = 0; d < D; ++d) n3 *= 3;
-    int cnt = 0;
-    for (int st = 0; st < n3; ++st) {
-      int s[D];
-      int rem = st;
-      bool skip = false;
-      if (order == 0) {
-#pragma unroll
-        for (int d = 0; d < D; ++d) { s[d] = rem % 3 - 1; rem /= 3; }
-      } else {
-#pragma unroll
-        for (int d = D - 1; d >= 0; --d) { s[d] = rem % 3 - 1; rem /= 3; }
-      }
-      i64 lin = 0, stride = 1;
-      u32 code = 0;
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        if (R.nc[d] == 1 && s[d] != 0) skip = true;
-        int c = ci[d] + s[d];
-        if (c < 0) c += R.nc[d];
-        else if (c >= R.nc[d]) c -= R.nc[d];
-        lin += (i64)c * stride;
-        stride *= R.nc[d];
-        code |= (u32)(s[d] + 1) << (2 * d);
-      }
-      if (skip) continue;
-      const i64 cell = R.c0 + lin;
-      const i32 lo = cstart[cell], hi = cstart[cell + 1];
-      for (i32 j = lo; j < hi; ++j) {
-        if ((i64)j == i) continue;
-        double ss = 0.0;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          double t = dp_sub(xi[d], x[(i64)j * D + d]);
-          t = dp_minimage_lt(t, R.L[d], R.halfL[d]);
-          ss = (d == 0) ? dp_mul(t, t) : dp_add(ss, dp_mul(t, t));
-        }
-        if (ss < R.nl_thresh) {
-          if (cnt < K) nl[(i64)cnt * Ns + i] = ((u32)j << (2 * D)) | code;
-          ++cnt;
-        }
+        for (int d = 0; d < D; ++d) sx[idx * D + d] = x[slot * D + d];
       }
     }
-    nlcnt[i] = cnt;
-    if (cnt > K) ctl->overflow = 1;
-    atomicMax(&ctl->max_count, cnt);
+    if (tid == 0) {
+      int mc = 0;
+      for (int u = 0; u < nh; ++u) mc = max(mc, S.hbase[u + 1] - S.hbase[u]);
+      if ((u32)mc > offmax + 1u) ctl->cell_overflow = 1;
+      atomicMax(&ctl->max_cell, mc);
+      atomicMax(&ctl->max_halo, S.nrec);
+    }
+    __syncthreads();
+    for (int k = wave; k < S.nhomep; k += nwaves) {
+      const int m = dp_home_of<D>(S, P.nhome, k);
+      const int uh = S.home_u[m];
+      const int self = S.hbase[uh] + (k - S.home_p[m]);
+      const i64 slot_i = S.hstart[uh] + (k - S.home_p[m]);
+      double xi[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) xi[d] = x[slot_i * D + d];
+      // lane r < NROW holds stencil row r: its valid cells are consecutive halo cells
+      int r_lo = 0, r_len = 0, r_u0 = 0;
+      if (lane < NROW) {
+        bool any = false;
+        int hi = 0;
+        for (int dx = 0; dx < 3; ++dx) {
+          const int st = 3 * lane + dx;
+          int sd[D];
+          dp_stencil_digits<D>(st, P.order, sd);
+          bool ok = true;
+#pragma unroll
+          for (int d = 0; d < D; ++d)
+            if (P.nc[d] == 1 && sd[d] != 0) ok = false;
+          if (!ok) continue;
+          const int u = uh + S.st_off[st];
+          if (!any) { any = true; r_u0 = u; r_lo = S.hbase[u]; }
+          hi = S.hbase[u + 1];
+        }
+        r_len = any ? hi - r_lo : 0;
+      }
+      int incl = r_len;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int vv = __shfl_up(incl, off);
+        if (lane >= off) incl += vv;
+      }
+      const int r_pref = incl - r_len;
+      const int ncand = __shfl(incl, NROW - 1);
+      int cnt = 0;
+      for (int base = 0; base < ncand; base += 64) {
+        const int c = base + lane;
+        // row of candidate c: largest r with r_pref[r] <= c (fixed-step search)
+        int row = 0;
+#pragma unroll
+        for (int step = 16; step >= 1; step >>= 1) {
+          const int cand = row + step;
+          const int pm = __shfl(r_pref, cand < NROW ? cand : NROW - 1);
+          if (cand < NROW && pm <= c) row = cand;
+        }
+        const int lo_row = __shfl(r_lo, row);
+        const int pf_row = __shfl(r_pref, row);
+        const int u0_row = __shfl(r_u0, row);
+        bool hit = false;
+        E ent = 0;
+        if (c < ncand) {
+          const int idx = lo_row + (c - pf_row);
+          int u = u0_row;
+          while (S.hbase[u + 1] <= idx) ++u;
+          const int st = 3 * row + dx0 + (u - u0_row);
+          const int off = idx - S.hbase[u];
+          double ss = 0.0;
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            const double xj = fits ? sx[idx * D + d] : x[((i64)S.hstart[u] + off) * D + d];
+            const double tt = dp_minimage_lt(dp_sub(xi[d], xj), P.L[d], P.halfL[d]);
+            ss = (d == 0) ? dp_mul(tt, tt) : dp_add(ss, dp_mul(tt, tt));
+          }
+          hit = (idx != self) && (ss < P.nl_thresh);
+          ent = (E)(((u32)st << ob) | ((u32)off & offmax));
+        }
+        const u64 mask = __ballot(hit);
+        const int pos = cnt + __popcll(mask & ((1ull << lane) - 1ull));
+        if (hit && pos < K) nl[slot_i * K + pos] = ent;
+        cnt += __popcll(mask);
+      }
+      if (lane == 0) {
+        nlcnt[slot_i] = cnt;
+        if (cnt > K) ctl->overflow = 1;
+        atomicMax(&ctl->max_count, cnt);
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -408,14 +470,37 @@ __global__ void k_from_orig(const T* src, T* dst, const i32* orig, i64 N, int W)
 }
 
 // Neighbour list in the reference layout: row orig[i], entries orig[j],
-// -1 padded to K (neighbor_list.py:14-17).
-__global__ void k_nl_export(const u32* nl, const i32* nlcnt, const i32* orig, i64 N, i64 Ns, int K, int sb,
-                            i32* out) {
+// -1 padded to K (neighbor_list.py:14-17).  Entries are decoded with the
+// particle's cell and the stencil.
+template <int D, typename E>
+__global__ void k_nl_export(const E* nl, const i32* nlcnt, const i32* orig, const i32* cell_of, const i32* cstart,
+                            const i32* rep, const DpRep* reps, i64 N, int K, int ob, i32* out) {
   DP_FOR(i, N) {
+    const DpRep& P = reps[rep[i]];
+    int ci[D];
+    dp_unlinear<D>((i64)cell_of[i] - P.c0, P.nc, P.order, ci);
     const i64 row = orig[i];
     const int cnt = nlcnt[i] < K ? nlcnt[i] : K;
-    for (int k = 0; k < K; ++k)
-      out[row * K + k] = (k < cnt) ? orig[nl[(i64)k * Ns + i] >> sb] : -1;
+    for (int k = 0; k < K; ++k) {
+      i32 val = -1;
+      if (k < cnt) {
+        const u32 e = (u32)nl[i * K + k];
+        const int st = (int)(e >> ob);
+        const int off = (int)(e & ((1u << ob) - 1u));
+        int sd[D], cc[D];
+        dp_stencil_digits<D>(st, P.order, sd);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          int c = ci[d] + (P.nc[d] == 1 ? 0 : sd[d]);
+          if (c < 0) c += P.nc[d];
+          if (c >= P.nc[d]) c -= P.nc[d];
+          cc[d] = c;
+        }
+        const i64 cell = P.c0 + dp_linear<D>(cc, P.nc, P.order);
+        val = orig[cstart[cell] + off];
+      }
+      out[row * K + k] = val;
+    }
   }
 }
 

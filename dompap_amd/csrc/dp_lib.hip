@@ -79,13 +79,26 @@ CodeCache& code_cache() {
 }
 
 
-std::string pair_source(int D, int mode, int ntypes, int even, int rs, int accumulate, const char* device_src) {
+struct PairConfig {
+  int D, mode, ntypes, even, rs, tpp, nlw, ob, pairprec;
+};
+
+// Stencil-index bits of a neighbour entry (3^D stencil cells).
+int stencil_bits(int D) {
+  int n3 = 1;
+  for (int d = 0; d < D; ++d) n3 *= 3;
+  int b = 0;
+  while ((1 << b) < n3) ++b;
+  return b;
+}
+
+std::string pair_source(const PairConfig& c, const char* device_src) {
   std::string src;
-  char buf[256];
+  char buf[512];
   snprintf(buf, sizeof(buf),
            "#define DP_D %d\n#define DP_MODE %d\n#define DP_NTYPES %d\n#define DP_EVEN %d\n#define DP_RS %d\n"
-           "#define DP_ACC %d\n",
-           D, mode, ntypes, even ? 1 : 0, rs, (mode == DP_MIXED) ? accumulate : 0);
+           "#define DP_TPP %d\n#define DP_NLW %d\n#define DP_OB %d\n#define DP_PAIRPREC %d\n",
+           c.D, c.mode, c.ntypes, c.even ? 1 : 0, c.rs, c.tpp, c.nlw, c.ob, c.pairprec);
   src += buf;
   src += "#include \"dp_device.h\"\n";
   src += device_src;
@@ -139,9 +152,9 @@ struct dp_ctx {
   int dev = 0;
   hipStream_t st = nullptr;
   int D = 3, mode = DP_FP64, R = 1, rs = 4;
-  i64 N = 0, Ns = 0;
+  i64 N = 0;
   int K = 0, order = DP_ORDER_X_FASTEST;
-  i64 Ctot = 0, ntiles = 0;
+  i64 Ctot = 0, ntiles_scan = 0;
   std::string err;
 
   // host mirrors of parameters
@@ -163,13 +176,25 @@ struct dp_ctx {
   u64* dispmax_d = nullptr;
   i64* nlupd_d = nullptr;
   double* x[2] = {nullptr, nullptr};
-  float* xs[2] = {nullptr, nullptr};
+  i32* xq[2] = {nullptr, nullptr};
   double *v = nullptr, *beta = nullptr, *mass = nullptr, *xref = nullptr, *F = nullptr;
   double *E = nullptr, *W = nullptr, *LAP = nullptr, *xi = nullptr, *red = nullptr;
   i32 *img = nullptr, *type = nullptr, *orig = nullptr, *rep = nullptr, *cellpack = nullptr;
   i32 *cell_of = nullptr, *rank = nullptr, *tmp = nullptr, *perm = nullptr, *nlcnt = nullptr;
   i32 *ccount = nullptr, *cstart = nullptr, *tiles = nullptr;
-  u32* nl = nullptr;
+  void* nl = nullptr;
+  size_t nl_bytes = 0;
+  // tile / entry-format plan (refresh_reps)
+  int nlw = 16, ob = 11, tpp = 8, qbits = 24, cap = 1024, cap_emit = 1024;
+  // tuning (0 = automatic)
+  int tune_tpp = 0, tune_tile = 0, fused = 1;
+  i64 ntiles = 0;
+  int tile_grid = 8;
+  size_t lds_pair = 0, lds_emit = 0;
+  // potential (compiled lazily once the plan is known)
+  std::string pot_src;
+  int pot_even = 0, pair_prec = 64;
+  std::string mod_key;
   char* scratch = nullptr;
   char* stage = nullptr;
   int cur = 0;
@@ -244,7 +269,28 @@ int release_graph(dp_ctx* ctx) {
   return DP_OK;
 }
 
-// Cells and replica table from box / cutoff.
+// Volume of the unit D-ball.
+double ball_volume(int D) {
+  const double pi = 3.14159265358979323846;
+  switch (D) {
+    case 1: return 2.0;
+    case 2: return pi;
+    case 3: return 4.0 * pi / 3.0;
+    case 4: return pi * pi / 2.0;
+  }
+  return 1.0;
+}
+
+// Cells, tiles, fixed-point scale, entry format and LDS capacity from the
+// boxes, cutoffs and particle counts.
+//  - cells: nc = floor(L / cutoff) per dimension (1 if fewer than 3 fit);
+//  - lanes per particle (tpp) from the expected list length;
+//  - tiles: enough home cells that one pass of the workgroup's lane groups
+//    is about full;
+//  - fixed-point scale: offsets from a cell centre and pair displacements
+//    stay below 2^30 units;
+//  - entries: 16-bit (stencil index, offset in cell) when cells are small
+//    enough, else 32-bit.
 int refresh_reps(dp_ctx* ctx) {
   if (!ctx->reps_dirty) return DP_OK;
   if (!ctx->have_box) return ctx->fail(DP_ERR_STATE, "dp_set_replicas has not been called");
@@ -252,120 +298,233 @@ int refresh_reps(dp_ctx* ctx) {
   const int D = ctx->D;
   const int bits = (D == 1) ? 31 : 32 / D;
   const i64 maxc = (D == 1) ? (i64)0x7fffffff : ((i64)1 << bits) - 1;
-  ctx->reps_h.assign(ctx->R, DpRep{});
-  i64 c0 = 0;
+  std::vector<DpRep> reps((size_t)ctx->R);
+  // expected list length -> lanes per particle
+  double kbar_max = 0.0;
   for (int r = 0; r < ctx->R; ++r) {
-    DpRep& P = ctx->reps_h[r];
+    double vol = 1.0;
+    for (int d = 0; d < D; ++d) vol *= ctx->box[(size_t)r * D + d];
+    const double n = (double)(ctx->offsets[r + 1] - ctx->offsets[r]);
+    const double cut = std::isfinite(ctx->cutoff[r]) ? ctx->cutoff[r] : 1e300;
+    const double kb = std::min(n, n / vol * ball_volume(D) * std::pow(std::max(cut, 0.0), D));
+    kbar_max = std::max(kbar_max, kb);
+  }
+  int tpp = 1;
+  if (kbar_max >= 160) tpp = 16;
+  else if (kbar_max >= 48) tpp = 8;
+  else if (kbar_max >= 20) tpp = 4;
+  else if (kbar_max >= 8) tpp = 2;
+  if (ctx->tune_tpp > 0) tpp = ctx->tune_tpp;
+  // home particles per tile: one pass of the workgroup's lane groups
+  const int target = ctx->tune_tile > 0 ? ctx->tune_tile : DP_TILE_THREADS / tpp;
+  i64 c0 = 0, t0 = 0;
+  double rmax = 1.0;   // largest |coordinate offset| or displacement, for the fixed-point scale
+  double occ_max = 0.0, halo_occ_max = 0.0;
+  for (int r = 0; r < ctx->R; ++r) {
+    DpRep& P = reps[r];
     std::memset(&P, 0, sizeof(P));
     P.p0 = ctx->offsets[r];
     P.n = ctx->offsets[r + 1] - ctx->offsets[r];
+    P.order = ctx->order;
     const double cut = ctx->cutoff[r];
     i64 nc[DP_DMAX];
     i64 ncell = 1;
     for (int d = 0; d < D; ++d) {
       const double L = ctx->box[(size_t)r * D + d];
-      if (!(L > 0.0) || !std::isfinite(L)) return ctx->fail(DP_ERR_ARG, "box length must be positive and finite");
       double q = (cut > 0.0 && std::isfinite(cut)) ? std::floor(L / cut) : 1.0;
       if (!(q >= 3.0)) q = 1.0;
       if (q > (double)maxc) q = (double)maxc;
       nc[d] = (i64)q;
       ncell *= nc[d];
     }
-    // keep the grid to at most ~8 cells per particle (sparse systems)
-    const i64 cap = std::max<i64>(64, 8 * std::max<i64>(P.n, 1));
-    while (ncell > cap) {
+    // sparse systems: at most ~8 cells per particle
+    const i64 cap_cells = std::max<i64>(64, 8 * std::max<i64>(P.n, 1));
+    while (ncell > cap_cells) {
+      int dm = 0;
+      for (int d = 1; d < D; ++d)
+        if (nc[d] > nc[dm]) dm = d;
+      if (nc[dm] <= 3) break;
+      nc[dm] = std::max<i64>(3, nc[dm] / 2);
       ncell = 1;
-      for (int d = 0; d < D; ++d) {
-        if (nc[d] > 1) nc[d] = std::max<i64>(3, nc[d] / 2);
-        if (nc[d] == 3 && ncell * 3 > cap) nc[d] = 1;
-        ncell *= nc[d];
-      }
-      bool all_small = true;
-      for (int d = 0; d < D; ++d) all_small &= (nc[d] <= 3);
-      if (all_small) break;
+      for (int d = 0; d < D; ++d) ncell *= nc[d];
     }
-    ncell = 1;
-    for (int d = 0; d < D; ++d) ncell *= nc[d];
-    P.c0 = c0;
-    P.ncell = ncell;
-    c0 += ncell;
-    P.collapsed_mask = 0;
+    const double occ = (double)P.n / (double)ncell;
+    // tile shape
+    i64 T[DP_DMAX];
+    for (int d = 0; d < D; ++d) T[d] = 1;
+    for (;;) {
+      i64 prodT = 1, prodH = 1;
+      for (int d = 0; d < D; ++d) {
+        prodT *= T[d];
+        prodH *= (nc[d] == 1) ? 1 : T[d] + 2;
+      }
+      int best = -1;
+      for (int d = 0; d < D; ++d) {
+        if (nc[d] == 1 || T[d] >= nc[d]) continue;
+        if (best < 0 || T[d] < T[best]) best = d;
+      }
+      if (best < 0) break;
+      const i64 nT = prodT / T[best] * (T[best] + 1);
+      const i64 nH = prodH / (T[best] + 2) * (T[best] + 3);
+      if (occ * (double)nT > 1.1 * target || nT > DP_HOME_MAX || nH > DP_HALO_MAX) break;
+      T[best] += 1;
+    }
+    i64 ntiles = 1, nhome = 1, nhalo = 1;
     for (int d = 0; d < D; ++d) {
       const double L = ctx->box[(size_t)r * D + d];
       P.nc[d] = (i32)nc[d];
+      P.T[d] = (i32)T[d];
+      P.nt[d] = (i32)((nc[d] + T[d] - 1) / T[d]);
+      P.Th[d] = (nc[d] == 1) ? 1 : (i32)T[d] + 2;
+      ntiles *= P.nt[d];
+      nhome *= P.T[d];
+      nhalo *= P.Th[d];
       P.L[d] = L;
       P.halfL[d] = L / 2;
       P.w[d] = L / (double)nc[d];
       P.inv_w[d] = (double)nc[d] / L;
-      const float hi = (float)P.w[d];
-      P.wf_hi[d] = hi;
-      P.wf_lo[d] = (float)(P.w[d] - (double)hi);
-      P.Lf[d] = (float)L;
-      P.invLf[d] = (float)(1.0 / L);
-      if (nc[d] == 1) P.collapsed_mask |= (1 << d);
+      if (nc[d] == 1) {
+        P.collapsed_mask |= (1 << d);
+        rmax = std::max(rmax, L);
+      } else {
+        rmax = std::max(rmax, 0.5 * P.w[d] + ctx->skin[r] + (std::isfinite(cut) ? cut : 0.0));
+      }
     }
+    P.c0 = c0;
+    P.ncell = ncell;
+    P.t0 = t0;
+    P.ntiles = ntiles;
+    P.nhome = (i32)nhome;
+    P.nhalo = (i32)nhalo;
+    c0 += ncell;
+    t0 += ntiles;
+    occ_max = std::max(occ_max, occ);
+    halo_occ_max = std::max(halo_occ_max, occ * (double)nhalo);
     P.nl_thresh = nl_threshold(cut);
     P.skin_half_sq = (0.5 * ctx->skin[r]) * (0.5 * ctx->skin[r]);
     P.dt = ctx->have_integ ? ctx->dt[r] : 0.01;
-    P.T = ctx->have_integ ? ctx->T[r] : 1.0;
+    P.T_ = ctx->have_integ ? ctx->T[r] : 1.0;
     P.tau = ctx->have_integ ? ctx->tau[r] : 0.1;
   }
   if (c0 >= (i64)0x7fffffff) return ctx->fail(DP_ERR_ARG, "too many cells (%lld)", (long long)c0);
+  // fixed-point scale: |values| < 2^30 units
+  int qbits = (int)std::floor(30.0 - std::log2(rmax));
+  if (qbits > 30) qbits = 30;
+  ctx->qbits = qbits;
+  const double qs = std::ldexp(1.0, qbits);
+  for (auto& P : reps)
+    for (int d = 0; d < D; ++d) {
+      P.Wq[d] = (i32)std::rint(P.w[d] * qs);
+      P.Lq[d] = (i32)std::min(std::rint(P.L[d] * qs), 2147483647.0);
+    }
+  // entry format
+  const int sbits = stencil_bits(D);
+  int nlw = 16, ob = 16 - sbits;
+  if (occ_max * 4.0 + 16.0 >= (double)(1 << ob)) {
+    nlw = 32;
+    ob = 32 - sbits;
+  }
+  // LDS records: 1.5x the mean halo, at most 48 KB
+  const size_t rec_bytes = (ctx->mode == DP_MIXED) ? (size_t)ctx->rs * 4 : (size_t)D * 8 + 4;
+  i64 cap = (i64)std::ceil(1.5 * halo_occ_max) + 64;
+  cap = std::min<i64>(cap, (i64)(48 * 1024 / rec_bytes));
+  cap = std::max<i64>(cap, 64);
+  const int old_nlw = ctx->nlw, old_tpp = ctx->tpp, old_ob = ctx->ob;
+  ctx->nlw = nlw;
+  ctx->ob = ob;
+  ctx->tpp = tpp;
+  ctx->cap = (int)cap;
+  ctx->lds_pair = (size_t)cap * rec_bytes;
+  {
+    i64 ce = (i64)std::ceil(1.5 * halo_occ_max) + 64;
+    ce = std::min<i64>(ce, (i64)(48 * 1024 / (D * 8)));
+    ctx->cap_emit = (int)std::max<i64>(ce, 64);
+  }
+  ctx->lds_emit = (size_t)ctx->cap_emit * D * 8;
+  ctx->ntiles = t0;
+  {
+    i64 g = std::min<i64>(t0, 4096);
+    g = std::max<i64>(8, (g + 7) / 8 * 8);
+    ctx->tile_grid = (int)g;
+  }
+  ctx->reps_h = reps;
   if (c0 != ctx->Ctot) {
     ctx->Ctot = c0;
-    ctx->ntiles = (c0 + DP_SCAN_TILE - 1) / DP_SCAN_TILE;
+    ctx->ntiles_scan = (c0 + DP_SCAN_TILE - 1) / DP_SCAN_TILE;
     CHK(dalloc(ctx, ctx->ccount, (size_t)c0));
     CHK(dalloc(ctx, ctx->cstart, (size_t)c0 + 1));
-    CHK(dalloc(ctx, ctx->tiles, (size_t)ctx->ntiles + 1));
+    CHK(dalloc(ctx, ctx->tiles, (size_t)ctx->ntiles_scan + 1));
+    release_graph(ctx);
+  }
+  const size_t nl_bytes = (size_t)ctx->N * (size_t)ctx->K * (size_t)(nlw / 8);
+  if (nl_bytes != ctx->nl_bytes || !ctx->nl) {
+    dfree(ctx->nl);
+    ctx->nl_bytes = 0;
+    if (hipMalloc(&ctx->nl, nl_bytes) != hipSuccess) {
+      ctx->nl = nullptr;
+      return ctx->fail(DP_ERR_HIP, "cannot allocate neighbour list of %zu bytes", nl_bytes);
+    }
+    ctx->nl_bytes = nl_bytes;
+    ctx->nl_built = false;
+    release_graph(ctx);
+  }
+  if (old_nlw != nlw || old_tpp != tpp || old_ob != ob) {
+    ctx->nl_built = false;
     release_graph(ctx);
   }
   HIPCHK(hipMemcpyAsync(ctx->reps_d, ctx->reps_h.data(), sizeof(DpRep) * ctx->R, hipMemcpyHostToDevice, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));  // reps_h may change on the next call
   ctx->reps_dirty = false;
-  ctx->tab_dirty = true;  // guard bands depend on cell widths
+  ctx->tab_dirty = true;  // guard bands depend on the fixed-point scale
   return DP_OK;
 }
 
-// Guard band for the FP32 cut test: bound on |s32 - s| for s near t.
+// Pair tables: exact FP64 cut thresholds and, for the mixed kernels, the
+// guard band of the FP32 screening of fixed-point squared distances.  The
+// fixed-point displacement differs from the exact one by at most 1.5 units
+// per dimension (two rounded offsets and a rounded cell width); the FP32 sum
+// of squares adds a few 2^-24 relative roundings.  Pairs inside the band are
+// decided with the reference's FP64 arithmetic.
 int refresh_tab(dp_ctx* ctx) {
   if (!ctx->tab_dirty) return DP_OK;
   if (!ctx->have_pot) return ctx->fail(DP_ERR_STATE, "no potential loaded (dp_load_potential)");
-  double xr_max = 0.0;
-  for (int r = 0; r < ctx->R; ++r) {
-    for (int d = 0; d < ctx->D; ++d) {
-      const DpRep& P = ctx->reps_h[r];
-      const double b = (P.nc[d] == 1) ? P.L[d] : 0.5 * P.w[d] + 0.5 * ctx->skin[r];
-      xr_max = std::max(xr_max, b);
-    }
-  }
   DpPairTab& T = ctx->tab_h;
   const int nt = ctx->ntypes;
+  const double qs = std::ldexp(1.0, ctx->qbits);
+  const double e32 = std::ldexp(1.0, -24);
+  T.qscale = 1.0 / qs;
+  T.qscale2 = 1.0 / (qs * qs);
+  T.qinv = qs;
   for (int a = 0; a < nt; ++a)
     for (int b = 0; b < nt; ++b) {
       const int k = a * nt + b;
+      const double s = T.sigma[k], e = T.eps[k];
+      T.inv_sigma2[k] = 1.0 / (s * s);
+      T.eps_over_sigma[k] = e / s;
+      T.inv_sigma2_qf[k] = (float)(T.inv_sigma2[k] / (qs * qs));
+      T.eps_over_sigma_qf[k] = (float)(T.eps_over_sigma[k] / qs);
+      T.eps_f[k] = (float)e;
+      T.sigma_f[k] = (float)s;
       const double t = T.cut_thresh[k];
-      T.thresh_f[k] = (float)t;
       if (!std::isfinite(t)) {
-        T.guard_lo_f[k] = INFINITY;
-        T.guard_hi_f[k] = INFINITY;
+        T.guard_lo_q[k] = INFINITY;
+        T.guard_hi_q[k] = INFINITY;
         continue;
       }
       if (t < 0) {
-        T.guard_lo_f[k] = -1.0f;
-        T.guard_hi_f[k] = -1.0f;
+        T.guard_lo_q[k] = -1.0f;
+        T.guard_hi_q[k] = -1.0f;
         continue;
       }
-      const double e32 = std::ldexp(1.0, -24);
-      const double st = std::sqrt(t);
-      const double dd = 4.0 * e32 * (xr_max + st);
-      double g = 2.0 * st * std::sqrt((double)ctx->D) * dd + 8.0 * e32 * t;
+      const double tq = t * qs * qs;
+      double g = 3.0 * std::sqrt((double)ctx->D) * std::sqrt(tq) + 2.25 * ctx->D + 8.0 * e32 * tq;
       g *= 4.0;  // safety factor
-      float lo = (float)(t - g), hi = (float)(t + g);
+      float lo = (float)(tq - g), hi = (float)(tq + g);
       lo = std::nextafter(lo, -INFINITY);
       hi = std::nextafter(hi, INFINITY);
       if (lo < 0) lo = 0;
-      T.guard_lo_f[k] = lo;
-      T.guard_hi_f[k] = hi;
+      T.guard_lo_q[k] = lo;
+      T.guard_hi_q[k] = hi;
     }
   HIPCHK(hipMemcpyAsync(ctx->tab_d, &ctx->tab_h, sizeof(DpPairTab), hipMemcpyHostToDevice, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
@@ -373,10 +532,35 @@ int refresh_tab(dp_ctx* ctx) {
   return DP_OK;
 }
 
+// Compile (or fetch) the pair module for the current plan.
+int ensure_module(dp_ctx* ctx) {
+  if (!ctx->have_pot) return ctx->fail(DP_ERR_STATE, "no potential loaded (dp_load_potential)");
+  PairConfig c{ctx->D, ctx->mode, ctx->ntypes, ctx->pot_even, ctx->rs, ctx->tpp, ctx->nlw, ctx->ob, ctx->pair_prec};
+  std::string src = pair_source(c, ctx->pot_src.c_str());
+  if (ctx->mod && src == ctx->mod_key) return DP_OK;
+  std::vector<char> code;
+  std::string log;
+  if (compile_cached(src, code, log) != DP_OK) return ctx->fail(DP_ERR_RTC, "hiprtc compile failed: %s", log.c_str());
+  release_graph(ctx);
+  if (ctx->mod) {
+    (void)hipModuleUnload(ctx->mod);
+    ctx->mod = nullptr;
+  }
+  HIPCHK(hipModuleLoadData(&ctx->mod, code.data()));
+  HIPCHK(hipModuleGetFunction(&ctx->f_force, ctx->mod, "dp_tile_force"));
+  HIPCHK(hipModuleGetFunction(&ctx->f_obs, ctx->mod, "dp_tile_obs"));
+  HIPCHK(hipModuleGetFunction(&ctx->f_gjf, ctx->mod, "dp_tile_gjf"));
+  HIPCHK(hipModuleGetFunction(&ctx->f_lf, ctx->mod, "dp_tile_lf"));
+  HIPCHK(hipModuleGetFunction(&ctx->f_int_gjf, ctx->mod, "dp_integrate_gjf"));
+  HIPCHK(hipModuleGetFunction(&ctx->f_int_lf, ctx->mod, "dp_integrate_lf"));
+  ctx->mod_key = src;
+  return DP_OK;
+}
+
 int prepare(dp_ctx* ctx) {
   CHK(refresh_reps(ctx));
   CHK(refresh_tab(ctx));
-  if (!ctx->mod) return ctx->fail(DP_ERR_STATE, "no potential loaded (dp_load_potential)");
+  CHK(ensure_module(ctx));
   return DP_OK;
 }
 
@@ -384,17 +568,20 @@ DpPairArgs pair_args(dp_ctx* ctx, int cur) {
   DpPairArgs a;
   std::memset(&a, 0, sizeof(a));
   a.x = ctx->x[cur];
-  a.xs = ctx->xs[cur];
+  a.xq = ctx->xq[cur];
   a.nl = ctx->nl;
   a.nlcnt = ctx->nlcnt;
   a.type = ctx->type;
-  a.rep = ctx->rep;
+  a.cstart = ctx->cstart;
   a.reps = ctx->reps_d;
   a.tab = ctx->tab_d;
   a.N = ctx->N;
-  a.Ns = ctx->Ns;
+  a.ntiles = ctx->ntiles;
+  a.K = ctx->K;
+  a.R = ctx->R;
+  a.cap = ctx->cap;
   a.x_out = ctx->x[1 - cur];
-  a.xs_out = ctx->xs[1 - cur];
+  a.xq_out = ctx->xq[1 - cur];
   a.v = ctx->v;
   a.beta = ctx->beta;
   a.img = ctx->img;
@@ -402,6 +589,7 @@ DpPairArgs pair_args(dp_ctx* ctx, int cur) {
   a.xref = ctx->xref;
   a.cellpack = ctx->cellpack;
   a.orig = ctx->orig;
+  a.rep = ctx->rep;
   a.xi = nullptr;
   a.seed = ctx->seed;
   a.ctl = ctx->ctl_d;
@@ -410,10 +598,19 @@ DpPairArgs pair_args(dp_ctx* ctx, int cur) {
   return a;
 }
 
+// Tile kernels: grid-stride over tiles with dynamic LDS for the staged halo.
 int launch_pair(dp_ctx* ctx, hipFunction_t f, DpPairArgs& a) {
-  const unsigned grid = (unsigned)((ctx->N + DP_BLOCK - 1) / DP_BLOCK);
   void* params[] = {&a};
-  HIPCHK(hipModuleLaunchKernel(f, grid, 1, 1, DP_BLOCK, 1, 1, 0, ctx->st, params, nullptr));
+  HIPCHK(hipModuleLaunchKernel(f, (unsigned)ctx->tile_grid, 1, 1, DP_TILE_THREADS, 1, 1,
+                               (unsigned)ctx->lds_pair, ctx->st, params, nullptr));
+  return DP_OK;
+}
+
+// Per-particle kernels (integrator with given forces).
+int launch_flat(dp_ctx* ctx, hipFunction_t f, DpPairArgs& a) {
+  const unsigned grid = (unsigned)((ctx->N + 255) / 256);
+  void* params[] = {&a};
+  HIPCHK(hipModuleLaunchKernel(f, grid, 1, 1, 256, 1, 1, 0, ctx->st, params, nullptr));
   return DP_OK;
 }
 
@@ -426,9 +623,10 @@ int launch_rebuild_t(dp_ctx* ctx) {
   k_fill_i32<<<grid_for(ctx->Ctot), DP_BLOCK, 0, st>>>(ctx->ccount, ctx->Ctot, 0, ctx->ctl_d);
   k_bin<D><<<gN, DP_BLOCK, 0, st>>>(ctx->x[cur], ctx->rep, ctx->reps_d, N, ctx->cell_of, ctx->rank, ctx->ccount,
                                      ctx->ctl_d);
-  k_scan_reduce<<<grid_for(ctx->ntiles * DP_BLOCK), DP_BLOCK, 0, st>>>(ctx->ccount, ctx->Ctot, ctx->tiles, ctx->ctl_d);
-  k_scan_top<<<1, DP_BLOCK, 0, st>>>(ctx->tiles, ctx->ntiles, ctx->ctl_d);
-  k_scan_down<<<grid_for(ctx->ntiles * DP_BLOCK), DP_BLOCK, 0, st>>>(ctx->ccount, ctx->Ctot, ctx->tiles, ctx->cstart,
+  k_scan_reduce<<<grid_for(ctx->ntiles_scan * DP_BLOCK), DP_BLOCK, 0, st>>>(ctx->ccount, ctx->Ctot, ctx->tiles,
+                                                                          ctx->ctl_d);
+  k_scan_top<<<1, DP_BLOCK, 0, st>>>(ctx->tiles, ctx->ntiles_scan, ctx->ctl_d);
+  k_scan_down<<<grid_for(ctx->ntiles_scan * DP_BLOCK), DP_BLOCK, 0, st>>>(ctx->ccount, ctx->Ctot, ctx->tiles, ctx->cstart,
                                                                     N, ctx->ctl_d);
   k_scatter<<<gN, DP_BLOCK, 0, st>>>(ctx->cell_of, ctx->rank, ctx->cstart, N, ctx->tmp, ctx->ctl_d);
   k_cellsort<<<gN, DP_BLOCK, 0, st>>>(ctx->tmp, ctx->cell_of, ctx->cstart, ctx->orig, N, ctx->perm, ctx->ctl_d);
@@ -436,10 +634,16 @@ int launch_rebuild_t(dp_ctx* ctx) {
                                          ctx->type, ctx->orig, ctx->cell_of, ctx->scratch, ctx->ctl_d);
   k_unpermute_store<D><<<gN, DP_BLOCK, 0, st>>>(N, ctx->x[cur], ctx->v, ctx->beta, ctx->mass, ctx->img, ctx->type,
                                                  ctx->orig, ctx->cell_of, ctx->scratch, ctx->xref, ctx->cellpack,
-                                                 ctx->xs[cur], ctx->rs, ctx->rep, ctx->reps_d, ctx->ctl_d);
-  k_nl_emit<D><<<grid_for(N, 8192), DP_BLOCK, 0, st>>>(ctx->x[cur], ctx->cellpack, ctx->cstart, ctx->rep,
-                                                       ctx->reps_d, N, ctx->Ns, ctx->K, ctx->order, ctx->nl,
-                                                       ctx->nlcnt, ctx->ctl_d);
+                                                 ctx->xq[cur], ctx->rs, std::ldexp(1.0, ctx->qbits), ctx->rep,
+                                                 ctx->reps_d, ctx->ctl_d);
+  if (ctx->nlw == 16)
+    k_emit_tile<D, u16><<<ctx->tile_grid, DP_TILE_THREADS, ctx->lds_emit, st>>>(
+        ctx->x[cur], ctx->cstart, ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, ctx->cap_emit, ctx->ob, (u16*)ctx->nl,
+        ctx->nlcnt, ctx->ctl_d);
+  else
+    k_emit_tile<D, u32><<<ctx->tile_grid, DP_TILE_THREADS, ctx->lds_emit, st>>>(
+        ctx->x[cur], ctx->cstart, ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, ctx->cap_emit, ctx->ob, (u32*)ctx->nl,
+        ctx->nlcnt, ctx->ctl_d);
   HIPCHK(hipGetLastError());
   return DP_OK;
 }
@@ -450,10 +654,6 @@ int launch_rebuild(dp_ctx* ctx) {
     case 2: return launch_rebuild_t<2>(ctx);
     case 3: return launch_rebuild_t<3>(ctx);
     case 4: return launch_rebuild_t<4>(ctx);
-    case 5: return launch_rebuild_t<5>(ctx);
-    case 6: return launch_rebuild_t<6>(ctx);
-    case 7: return launch_rebuild_t<7>(ctx);
-    case 8: return launch_rebuild_t<8>(ctx);
   }
   return ctx->fail(DP_ERR_ARG, "dimension %d unsupported", ctx->D);
 }
@@ -473,10 +673,6 @@ int launch_skin_check(dp_ctx* ctx) {
     case 2: return launch_skin_check_t<2>(ctx);
     case 3: return launch_skin_check_t<3>(ctx);
     case 4: return launch_skin_check_t<4>(ctx);
-    case 5: return launch_skin_check_t<5>(ctx);
-    case 6: return launch_skin_check_t<6>(ctx);
-    case 7: return launch_skin_check_t<7>(ctx);
-    case 8: return launch_skin_check_t<8>(ctx);
   }
   return ctx->fail(DP_ERR_ARG, "dimension %d unsupported", ctx->D);
 }
@@ -485,7 +681,8 @@ template <int D>
 int launch_records_t(dp_ctx* ctx) {
   if (ctx->mode != DP_MIXED || !ctx->nl_built) return DP_OK;
   k_make_records<D><<<grid_for(ctx->N), DP_BLOCK, 0, ctx->st>>>(ctx->N, ctx->x[ctx->cur], ctx->type, ctx->cellpack,
-                                                                ctx->xs[ctx->cur], ctx->rs, ctx->rep, ctx->reps_d);
+                                                                ctx->xq[ctx->cur], ctx->rs, std::ldexp(1.0, ctx->qbits),
+                                                                ctx->rep, ctx->reps_d);
   HIPCHK(hipGetLastError());
   return DP_OK;
 }
@@ -496,10 +693,6 @@ int launch_records(dp_ctx* ctx) {
     case 2: return launch_records_t<2>(ctx);
     case 3: return launch_records_t<3>(ctx);
     case 4: return launch_records_t<4>(ctx);
-    case 5: return launch_records_t<5>(ctx);
-    case 6: return launch_records_t<6>(ctx);
-    case 7: return launch_records_t<7>(ctx);
-    case 8: return launch_records_t<8>(ctx);
   }
   return ctx->fail(DP_ERR_ARG, "dimension %d unsupported", ctx->D);
 }
@@ -514,19 +707,25 @@ int launch_prologue(dp_ctx* ctx, int advance) {
 int check_ctl(dp_ctx* ctx) {
   HIPCHK(hipMemcpyAsync(ctx->ctl_h, ctx->ctl_d, sizeof(DpCtl), hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
-  if (ctx->ctl_h->overflow || ctx->ctl_h->nonfinite) {
+  if (ctx->ctl_h->overflow || ctx->ctl_h->nonfinite || ctx->ctl_h->cell_overflow) {
     const bool ov = ctx->ctl_h->overflow != 0;
+    const bool co = ctx->ctl_h->cell_overflow != 0;
     const int mc = ctx->ctl_h->max_count;
     // clear the sticky flags
     DpCtl c = *ctx->ctl_h;
     c.overflow = 0;
     c.nonfinite = 0;
+    c.cell_overflow = 0;
     HIPCHK(hipMemcpy(ctx->ctl_d, &c, sizeof(DpCtl), hipMemcpyHostToDevice));
     if (ov)
       return ctx->fail(DP_ERR_OVERFLOW,
                        "Number of neighbors (%d) exceeds max_number_of_neighbors=%d; "
                        "increase max_number_of_neighbors",
                        mc, ctx->K);
+    if (co)
+      return ctx->fail(DP_ERR_OVERFLOW,
+                       "a cell holds %d particles, more than the %d-bit neighbour entries can address",
+                       ctx->ctl_h->max_cell, ctx->nlw);
     return ctx->fail(DP_ERR_NONFINITE, "non-finite particle position encountered");
   }
   return DP_OK;
@@ -567,9 +766,14 @@ int launch_one_step(dp_ctx* ctx, int integ, const double* xi_dev, const std::pai
   CHK(launch_rebuild(ctx));
   DpPairArgs a = pair_args(ctx, ctx->cur);
   a.xi = xi_dev;
-  hipFunction_t f = (integ == DP_INTEGRATOR_GJF) ? ctx->f_gjf : ctx->f_lf;
   if (ev) HIPCHK(hipEventRecord(ev->first, ctx->st));
-  CHK(launch_pair(ctx, f, a));
+  if (ctx->fused) {
+    CHK(launch_pair(ctx, (integ == DP_INTEGRATOR_GJF) ? ctx->f_gjf : ctx->f_lf, a));
+  } else {
+    a.F = ctx->F;
+    CHK(launch_pair(ctx, ctx->f_force, a));
+    CHK(launch_flat(ctx, (integ == DP_INTEGRATOR_GJF) ? ctx->f_int_gjf : ctx->f_int_lf, a));
+  }
   if (ev) HIPCHK(hipEventRecord(ev->second, ctx->st));
   ctx->cur = 1 - ctx->cur;
   return DP_OK;
@@ -632,11 +836,18 @@ int capture_graph(dp_ctx* ctx, int integ) {
 DP_API int dp_version(void) { return 100; }
 
 
-DP_API int dp_compile_check(int dim, int precision, int ntypes, int even, const char* device_src, int accumulate,
+// Default plan used by the device-free compile checks.
+PairConfig default_config(int dim, int precision, int ntypes, int even, int pair_precision) {
+  PairConfig c{dim, precision, ntypes, even, rs_for_dim(dim), 8, 16, 16 - stencil_bits(dim),
+               pair_precision ? 32 : 64};
+  return c;
+}
+
+DP_API int dp_compile_check(int dim, int precision, int ntypes, int even, const char* device_src, int pair_precision,
                             char* log, int64_t log_len, int64_t* code_size) {
   if (dim < 1 || dim > DP_DMAX || !device_src) return DP_ERR_ARG;
   if (ntypes < 1 || ntypes > DP_MAXTYPES) return DP_ERR_ARG;
-  std::string src = pair_source(dim, precision, ntypes, even, rs_for_dim(dim), accumulate, device_src);
+  std::string src = pair_source(default_config(dim, precision, ntypes, even, pair_precision), device_src);
   std::vector<char> code;
   std::string lg;
   int rc = compile_cached(src, code, lg);
@@ -648,10 +859,11 @@ DP_API int dp_compile_check(int dim, int precision, int ntypes, int even, const 
   return rc;
 }
 
-DP_API int dp_compile_code(int dim, int precision, int ntypes, int even, const char* device_src, int accumulate,
+DP_API int dp_compile_code(int dim, int precision, int ntypes, int even, const char* device_src, int pair_precision,
                            char* out, int64_t out_len, int64_t* code_size) {
   if (dim < 1 || dim > DP_DMAX || !device_src) return DP_ERR_ARG;
-  std::string src = pair_source(dim, precision, ntypes, even, rs_for_dim(dim), accumulate, device_src);
+  if (ntypes < 1 || ntypes > DP_MAXTYPES) return DP_ERR_ARG;
+  std::string src = pair_source(default_config(dim, precision, ntypes, even, pair_precision), device_src);
   std::vector<char> code;
   std::string lg;
   int rc = compile_cached(src, code, lg);
@@ -699,7 +911,6 @@ DP_API int dp_create(dp_ctx** out, int device, int dim, int64_t n_total, int n_r
   ctx->dev = device;
   ctx->D = dim;
   ctx->N = n_total;
-  ctx->Ns = (n_total + 63) / 64 * 64;
   ctx->R = n_replicas;
   ctx->mode = precision;
   ctx->rs = rs_for_dim(dim);
@@ -727,8 +938,8 @@ DP_API int dp_create(dp_ctx** out, int device, int dim, int64_t n_total, int n_r
   CCHK(dalloc(ctx, ctx->x[0], (size_t)(N * D)));
   CCHK(dalloc(ctx, ctx->x[1], (size_t)(N * D)));
   if (precision == DP_MIXED) {
-    CCHK(dalloc(ctx, ctx->xs[0], (size_t)(N * ctx->rs)));
-    CCHK(dalloc(ctx, ctx->xs[1], (size_t)(N * ctx->rs)));
+    CCHK(dalloc(ctx, ctx->xq[0], (size_t)(N * ctx->rs)));
+    CCHK(dalloc(ctx, ctx->xq[1], (size_t)(N * ctx->rs)));
   }
   CCHK(dalloc(ctx, ctx->v, (size_t)(N * D)));
   CCHK(dalloc(ctx, ctx->beta, (size_t)(N * D)));
@@ -799,7 +1010,7 @@ DP_API int dp_destroy(dp_ctx* ctx) {
   for (auto& ev : ctx->ev_free) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
   if (ctx->mod) (void)hipModuleUnload(ctx->mod);
   dfree(ctx->reps_d); dfree(ctx->tab_d); dfree(ctx->ctl_d); dfree(ctx->dispmax_d); dfree(ctx->nlupd_d);
-  dfree(ctx->red); dfree(ctx->x[0]); dfree(ctx->x[1]); dfree(ctx->xs[0]); dfree(ctx->xs[1]);
+  dfree(ctx->red); dfree(ctx->x[0]); dfree(ctx->x[1]); dfree(ctx->xq[0]); dfree(ctx->xq[1]);
   dfree(ctx->v); dfree(ctx->beta); dfree(ctx->mass); dfree(ctx->xref); dfree(ctx->F); dfree(ctx->E);
   dfree(ctx->W); dfree(ctx->LAP); dfree(ctx->xi); dfree(ctx->img); dfree(ctx->type); dfree(ctx->orig);
   dfree(ctx->rep); dfree(ctx->cellpack); dfree(ctx->cell_of); dfree(ctx->rank); dfree(ctx->tmp);
@@ -861,13 +1072,7 @@ DP_API int dp_set_neighbor_params(dp_ctx* ctx, const double* cutoff, const doubl
   ctx->cutoff.assign(cutoff, cutoff + R);
   ctx->skin.assign(skin, skin + R);
   ctx->order = order;
-  if (max_neighbors != ctx->K || !ctx->nl) {
-    const size_t bytes = (size_t)max_neighbors * (size_t)ctx->Ns * 4;
-    dfree(ctx->nl);
-    if (hipMalloc((void**)&ctx->nl, bytes) != hipSuccess) {
-      ctx->K = 0;
-      return ctx->fail(DP_ERR_HIP, "cannot allocate neighbour list of %zu bytes", bytes);
-    }
+  if (max_neighbors != ctx->K) {
     ctx->K = max_neighbors;
     release_graph(ctx);
   }
@@ -878,47 +1083,35 @@ DP_API int dp_set_neighbor_params(dp_ctx* ctx, const double* cutoff, const doubl
 }
 
 DP_API int dp_load_potential(dp_ctx* ctx, const char* device_src, int even, double r_cut, double u_rc, int ntypes,
-                             const double* sigma, const double* eps, int accumulate) {
+                             const double* sigma, const double* eps, int pair_precision) {
   if (!ctx || !device_src || !sigma || !eps) return DP_ERR_ARG;
   (void)hipSetDevice(ctx->dev);
   if (ntypes < 1 || ntypes > DP_MAXTYPES) return ctx->fail(DP_ERR_ARG, "ntypes must be in 1..%d", DP_MAXTYPES);
-  if (accumulate != 0 && accumulate != 1) return ctx->fail(DP_ERR_ARG, "accumulate must be 0 or 1");
+  if (pair_precision != 0 && pair_precision != 1) return ctx->fail(DP_ERR_ARG, "pair_precision must be 0 or 1");
   for (int k = 0; k < ntypes * ntypes; ++k)
     if (!(sigma[k] > 0.0)) return ctx->fail(DP_ERR_ARG, "sigma must be positive");
-  std::string src = pair_source(ctx->D, ctx->mode, ntypes, even, ctx->rs, accumulate, device_src);
-  std::vector<char> code;
-  std::string log;
-  if (compile_cached(src, code, log) != DP_OK) return ctx->fail(DP_ERR_RTC, "hiprtc compile failed: %s", log.c_str());
-  release_graph(ctx);
-  if (ctx->mod) {
-    (void)hipModuleUnload(ctx->mod);
-    ctx->mod = nullptr;
+  // compile now (with the current plan) so that errors surface here
+  {
+    PairConfig c = default_config(ctx->D, ctx->mode, ntypes, even, pair_precision);
+    std::string src = pair_source(c, device_src);
+    std::vector<char> code;
+    std::string log;
+    if (compile_cached(src, code, log) != DP_OK)
+      return ctx->fail(DP_ERR_RTC, "hiprtc compile failed: %s", log.c_str());
   }
-  HIPCHK(hipModuleLoadData(&ctx->mod, code.data()));
-  HIPCHK(hipModuleGetFunction(&ctx->f_force, ctx->mod, "dp_pair_force"));
-  HIPCHK(hipModuleGetFunction(&ctx->f_obs, ctx->mod, "dp_pair_obs"));
-  HIPCHK(hipModuleGetFunction(&ctx->f_gjf, ctx->mod, "dp_step_gjf"));
-  HIPCHK(hipModuleGetFunction(&ctx->f_lf, ctx->mod, "dp_step_lf"));
-  HIPCHK(hipModuleGetFunction(&ctx->f_int_gjf, ctx->mod, "dp_integrate_gjf"));
-  HIPCHK(hipModuleGetFunction(&ctx->f_int_lf, ctx->mod, "dp_integrate_lf"));
-  // tables
+  ctx->pot_src = device_src;
+  ctx->pot_even = even ? 1 : 0;
+  ctx->pair_prec = pair_precision ? 32 : 64;
+  release_graph(ctx);
   DpPairTab& T = ctx->tab_h;
   std::memset(&T, 0, sizeof(T));
   ctx->ntypes = ntypes;
   for (int a = 0; a < ntypes; ++a)
     for (int b = 0; b < ntypes; ++b) {
       const int k = a * ntypes + b;
-      const double s = sigma[k], e = eps[k];
-      T.sigma[k] = s;
-      T.eps[k] = e;
-      T.cut_thresh[k] = cut_threshold(s, r_cut);
-      T.inv_sigma[k] = 1.0 / s;
-      T.inv_sigma2[k] = 1.0 / (s * s);
-      T.sigma_f[k] = (float)s;
-      T.eps_f[k] = (float)e;
-      T.eps_over_sigma_f[k] = (float)(e / s);
-      T.inv_sigma_f[k] = (float)(1.0 / s);
-      T.inv_sigma2_f[k] = (float)(1.0 / (s * s));
+      T.sigma[k] = sigma[k];
+      T.eps[k] = eps[k];
+      T.cut_thresh[k] = cut_threshold(sigma[k], r_cut);
     }
   T.r_cut = r_cut;
   T.u_rc = u_rc;
@@ -1123,11 +1316,36 @@ DP_API int dp_integrate(dp_ctx* ctx, const double* forces, int integrator, const
   a.F = ctx->F;
   a.xi = xi ? ctx->xi : nullptr;
   a.wrap = wrap ? 1 : 0;
-  CHK(launch_pair(ctx, integrator == DP_INTEGRATOR_GJF ? ctx->f_int_gjf : ctx->f_int_lf, a));
+  CHK(launch_flat(ctx, integrator == DP_INTEGRATOR_GJF ? ctx->f_int_gjf : ctx->f_int_lf, a));
   ctx->cur = 1 - ctx->cur;
   ctx->pos_dirty = true;  // the list's skin check must see the new positions
   CHK(launch_records(ctx));
   HIPCHK(hipStreamSynchronize(ctx->st));
+  return DP_OK;
+}
+
+DP_API int dp_set_tuning(dp_ctx* ctx, int threads_per_particle, int tile_particles, int fused) {
+  if (!ctx) return DP_ERR_ARG;
+  if (threads_per_particle != 0 && threads_per_particle != 1 && threads_per_particle != 2 &&
+      threads_per_particle != 4 && threads_per_particle != 8 && threads_per_particle != 16)
+    return ctx->fail(DP_ERR_ARG, "threads_per_particle must be 0 (automatic), 1, 2, 4, 8 or 16");
+  if (tile_particles < 0) return ctx->fail(DP_ERR_ARG, "tile_particles must be >= 0");
+  ctx->tune_tpp = threads_per_particle;
+  ctx->tune_tile = tile_particles;
+  ctx->fused = fused ? 1 : 0;
+  ctx->reps_dirty = true;
+  release_graph(ctx);
+  return DP_OK;
+}
+
+DP_API int dp_get_plan(dp_ctx* ctx, int32_t* out, int n) {
+  if (!ctx || !out) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  CHK(refresh_reps(ctx));
+  const int32_t v[12] = {ctx->tpp,  ctx->nlw, ctx->ob, ctx->qbits, ctx->cap, ctx->cap_emit,
+                         (int32_t)ctx->ntiles, ctx->tile_grid, ctx->fused, ctx->reps_h[0].nhome,
+                         ctx->reps_h[0].nhalo, (int32_t)ctx->Ctot};
+  for (int k = 0; k < n && k < 12; ++k) out[k] = v[k];
   return DP_OK;
 }
 
@@ -1143,6 +1361,31 @@ DP_API int dp_set_seed(dp_ctx* ctx, uint64_t seed, uint64_t step_counter) {
   return DP_OK;
 }
 
+namespace {
+template <int D>
+int launch_export_t(dp_ctx* ctx, i32* out) {
+  const int g = grid_for(ctx->N);
+  if (ctx->nlw == 16)
+    k_nl_export<D, u16><<<g, DP_BLOCK, 0, ctx->st>>>((const u16*)ctx->nl, ctx->nlcnt, ctx->orig, ctx->cell_of,
+                                                     ctx->cstart, ctx->rep, ctx->reps_d, ctx->N, ctx->K, ctx->ob, out);
+  else
+    k_nl_export<D, u32><<<g, DP_BLOCK, 0, ctx->st>>>((const u32*)ctx->nl, ctx->nlcnt, ctx->orig, ctx->cell_of,
+                                                     ctx->cstart, ctx->rep, ctx->reps_d, ctx->N, ctx->K, ctx->ob, out);
+  HIPCHK(hipGetLastError());
+  return DP_OK;
+}
+
+int launch_export(dp_ctx* ctx, i32* out) {
+  switch (ctx->D) {
+    case 1: return launch_export_t<1>(ctx, out);
+    case 2: return launch_export_t<2>(ctx, out);
+    case 3: return launch_export_t<3>(ctx, out);
+    case 4: return launch_export_t<4>(ctx, out);
+  }
+  return ctx->fail(DP_ERR_ARG, "dimension %d unsupported", ctx->D);
+}
+}  // namespace
+
 DP_API int dp_get_neighbor_list(dp_ctx* ctx, int32_t* out) {
   if (!ctx || !out) return DP_ERR_ARG;
   (void)hipSetDevice(ctx->dev);
@@ -1150,8 +1393,7 @@ DP_API int dp_get_neighbor_list(dp_ctx* ctx, int32_t* out) {
   const size_t bytes = (size_t)ctx->N * ctx->K * 4;
   i32* tmpbuf = nullptr;
   HIPCHK(hipMalloc((void**)&tmpbuf, bytes));
-  k_nl_export<<<grid_for(ctx->N), DP_BLOCK, 0, ctx->st>>>(ctx->nl, ctx->nlcnt, ctx->orig, ctx->N, ctx->Ns, ctx->K,
-                                                          2 * ctx->D, tmpbuf);
+  CHK(launch_export(ctx, tmpbuf));
   hipError_t e = hipMemcpyAsync(out, tmpbuf, bytes, hipMemcpyDefault, ctx->st);
   hipError_t e2 = hipStreamSynchronize(ctx->st);
   (void)hipFree(tmpbuf);

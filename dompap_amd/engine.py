@@ -107,19 +107,34 @@ class Engine:
         self.max_neighbors = int(max_neighbors)
 
     def load_potential(self, code: PotentialCode, r_cut: float, u_rc: float, sigma, epsilon,
-                       accumulate: str = "fp64") -> None:
+                       pair_precision: str = "fp64") -> None:
+        """pair_precision (mixed mode only): "fp64" evaluates in-range pair
+        terms in FP64 (default), "fp32" in FP32; the candidate screening and
+        the displacements are fixed-point / FP32 in both cases."""
         sig = np.ascontiguousarray(np.atleast_2d(np.asarray(sigma, dtype=np.float64)))
         eps = np.ascontiguousarray(np.atleast_2d(np.asarray(epsilon, dtype=np.float64)))
         nt = sig.shape[0]
         if sig.shape != (nt, nt) or eps.shape != (nt, nt):
             raise ValueError("sigma/epsilon tables must be square and of equal size")
-        acc = {"fp64": 0, "kahan": 1}[accumulate]
+        acc = {"fp64": 0, "fp32": 1}[pair_precision]
         rc = self.lib.dp_load_potential(self.ctx, code.source.encode(), int(code.even), float(r_cut),
                                         float(u_rc), nt, _ptr(sig), _ptr(eps), acc)
         self._chk(rc)
 
     def set_seed(self, seed: int, step_counter: int = 0) -> None:
         self._chk(self.lib.dp_set_seed(self.ctx, int(seed) & (2**64 - 1), int(step_counter)))
+
+    def set_tuning(self, threads_per_particle: int = 0, tile_particles: int = 0, fused: bool = True) -> None:
+        self._chk(self.lib.dp_set_tuning(self.ctx, int(threads_per_particle), int(tile_particles), int(bool(fused))))
+
+    PLAN_KEYS = ("threads_per_particle", "entry_bits", "offset_bits", "fixed_point_bits", "lds_records",
+                 "lds_records_emit", "tiles", "tile_grid", "fused", "home_cells_per_tile", "halo_cells_per_tile",
+                 "cells")
+
+    def plan(self) -> dict:
+        out = np.zeros(len(self.PLAN_KEYS), dtype=np.int32)
+        self._chk(self.lib.dp_get_plan(self.ctx, _ptr(out), len(out)))
+        return dict(zip(self.PLAN_KEYS, (int(v) for v in out)))
 
     def set_graph_block(self, steps: int) -> None:
         self._chk(self.lib.dp_set_graph_block(self.ctx, int(steps)))

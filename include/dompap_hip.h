@@ -61,7 +61,7 @@ typedef struct dp_ctx dp_ctx;
 int dp_version(void);
 
 /* Create a context for n_total particles in n_replicas systems of dimension
- * dim (1..8) on HIP device `device`.  Replaces the state storage of the
+ * dim (1..4) on HIP device `device`.  Replaces the state storage of the
  * Simulation dataclass (dompap/Simulation.py:101-147). */
 int dp_create(dp_ctx** out, int device, int dim, int64_t n_total, int n_replicas, int precision);
 int dp_destroy(dp_ctx* ctx);
@@ -86,11 +86,12 @@ int dp_set_neighbor_params(dp_ctx* ctx, const double* cutoff, const double* skin
 /* Compile (hiprtc, gfx950) the pair kernels for a SymPy-generated potential.
  * device_src defines dp_pot_u/f/c (and dp_pot_gs/us when even != 0);
  * sigma/eps are ntypes*ntypes tables; u_rc = u(r_cut) (energy shift).
- * accumulate: 0 = FP64 accumulators, 1 = FP32 Kahan (mixed mode only).
+ * pair_precision (DP_MIXED only): 0 = in-range pair terms in FP64 from the
+ * exact fixed-point displacement, 1 = in FP32; forces accumulate in FP64.
  * Replaces make_pair_potential (potential.py:43-66) and the sigma/epsilon
  * callables of set_pair_potential_parameters (Simulation.py:469-525). */
 int dp_load_potential(dp_ctx* ctx, const char* device_src, int even, double r_cut, double u_rc, int ntypes,
-                      const double* sigma, const double* eps, int accumulate);
+                      const double* sigma, const double* eps, int pair_precision);
 
 /* Upload / download one per-particle field in the reference layout.
  * Simulation attribute assignment / access (Simulation.py:101-147). */
@@ -129,11 +130,23 @@ int dp_step(dp_ctx* ctx, int64_t nsteps, int integrator, const double* xi);
  * dp_step. */
 int dp_integrate(dp_ctx* ctx, const double* forces, int integrator, const double* xi, int wrap);
 
+/* Kernel tuning (no counterpart in the reference's C surface; the GPU
+ * autotuner replacing dompap/tools/autotune.py:4-178 drives it): lanes per particle
+ * (0 = automatic, else 1/2/4/8/16), home particles per tile (0 = automatic),
+ * fused force+integrate kernel (1) or force kernel then integrate kernel (0). */
+int dp_set_tuning(dp_ctx* ctx, int threads_per_particle, int tile_particles, int fused);
+/* Current plan: tpp, entry bits, offset bits, fixed-point bits, LDS record
+ * capacity (pair, emit), tiles, tile grid, fused, home / halo cells per tile
+ * of replica 0, cells (first n values). */
+int dp_get_plan(dp_ctx* ctx, int32_t* out, int n);
+
 /* RNG key and step counter. */
 int dp_set_seed(dp_ctx* ctx, uint64_t seed, uint64_t step_counter);
 
 /* Neighbour list in the reference layout int32 [N][K], -1 padded
- * (Simulation.neighbor_list); rows in stencil order, members in ascending id. */
+ * (Simulation.neighbor_list); rows in stencil order, members in ascending id.
+ * (On the device the list is kept as 16- or 32-bit (stencil cell, offset in
+ * cell) entries over cell-sorted particles.) */
 int dp_get_neighbor_list(dp_ctx* ctx, int32_t* out);
 /* Row lengths int32 [N]: get_number_of_neighbors (neighbor_list.py:204-207). */
 int dp_get_neighbor_counts(dp_ctx* ctx, int32_t* out);
@@ -155,11 +168,11 @@ int dp_get_kernel_time(dp_ctx* ctx, double* total_ms, int64_t* launches);
  * gfx950): build-time check of generated code.  log receives the compiler
  * log (truncated to log_len), code_size the code-object size.  Part of
  * make_pair_potential's replacement (potential.py:43-66). */
-int dp_compile_check(int dim, int precision, int ntypes, int even, const char* device_src, int accumulate,
+int dp_compile_check(int dim, int precision, int ntypes, int even, const char* device_src, int pair_precision,
                      char* log, int64_t log_len, int64_t* code_size);
 /* Same compile; copies the gfx950 code object into out when out_len is large
  * enough (code_size receives the size). */
-int dp_compile_code(int dim, int precision, int ntypes, int even, const char* device_src, int accumulate,
+int dp_compile_code(int dim, int precision, int ntypes, int even, const char* device_src, int pair_precision,
                     char* out, int64_t out_len, int64_t* code_size);
 
 /* Block until all work on the context stream is done. */

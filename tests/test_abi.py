@@ -36,6 +36,7 @@ def test_errors_without_device_are_reported():
         return
     assert rc < 0 and lib.dp_last_error(None)
     assert lib.dp_create(ctypes.byref(ctx), 0, 9, 10, 1, 0) == _lib.DP_ERR_ARG
+    assert lib.dp_create(ctypes.byref(ctx), 0, 5, 10, 1, 0) == _lib.DP_ERR_ARG
 
 
 @pytest.mark.parametrize("pot", ["4*(r**-12-r**-6)", "(1-r)**2", "Harmonic repulsive", "r**-12",
@@ -43,7 +44,7 @@ def test_errors_without_device_are_reported():
 def test_pair_kernels_compile(pot):
     lib = _lib.load()
     code = codegen.generate(pot)
-    for dim in (2, 3, 4):
+    for dim in (1, 2, 3, 4):
         for mode in (0, 1):
             log = ctypes.create_string_buffer(1 << 15)
             size = ctypes.c_int64()
