@@ -31,7 +31,19 @@ template <int N, class T> __device__ __forceinline__ T dp_ipow(T x) {
   else return x * dp_ipow<N - 1, T>(x);
 }
 __device__ __forceinline__ float dp_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+#if DP_MODE == 1
+// mixed mode: hardware reciprocal refined by two Newton steps (within an ulp
+// for normal arguments) instead of the correctly rounded quotient
+__device__ __forceinline__ double dp_rcp(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+#else
 __device__ __forceinline__ double dp_rcp(double x) { return 1.0 / x; }
+#endif
 __device__ __forceinline__ float dp_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double dp_sqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float dp_pow(float a, float b) { return powf(a, b); }

@@ -54,19 +54,31 @@ struct DpRep {
   i32 order;            // 0: dimension 0 fastest, 1: dimension D-1 fastest
 };
 
-// Device control block (one per context).
+// Device control block (one per context).  The neighbour-list statistics are
+// kept in DP_STAT_SLOTS slots, one 128-byte line each, that workgroups update
+// by blockIdx % DP_STAT_SLOTS: same-address device atomics from thousands of
+// workgroups serialise, spread ones do not.  The host reduces the slots into
+// max_count / max_cell / max_halo after each read-back.
+#define DP_STAT_SLOTS 64
+#define DP_STAT_STRIDE 32
 struct DpCtl {
   u64 step_counter;     // steps taken by the device (RNG counter)
   u64 step_index;       // counter value used by the current step
   i32 rebuild;          // 1 if the current step rebuilds the neighbour list
   i32 force_rebuild;    // host request
+  i32 need_rebuild;     // some particle moved more than half the skin since the last build
   i32 overflow;         // some row needed more than K entries
-  i32 max_count;        // largest row length seen at the last emit
   i32 nonfinite;        // a non-finite position was seen while binning
   i32 cell_overflow;    // a cell held more particles than the entry format allows
-  i32 max_cell;         // largest cell occupancy at the last rebuild
-  i32 max_halo;         // largest staged halo (records) at the last rebuild
+  i32 max_count;        // host-side reduction of the slots: largest row length
+  i32 max_cell;         //   largest cell occupancy at the last rebuild
+  i32 max_halo;         //   largest staged halo (records) at the last rebuild
+  i32 pad_[5];
+  i32 slot[DP_STAT_SLOTS * DP_STAT_STRIDE];  // [s * stride + 0 / 1 / 2]: the three maxima
 };
+#define DP_STAT_COUNT 0
+#define DP_STAT_CELL 1
+#define DP_STAT_HALO 2
 
 // Pair-interaction tables (per context; indexed [ti * ntypes + tj]).
 struct DpPairTab {
@@ -103,7 +115,8 @@ struct DpPairArgs {
   i64 ntiles;             // total tiles (grid size)
   i32 K, R;
   i32 cap;                // records that fit in the workgroup's LDS
-  i32 pad0_;
+  i32 tstride;            // ints per tile in tiletab
+  const i32* tiletab;     // [ntiles * tstride] tile tables written at the last build
   // observable outputs (nullable)
   double* F;              // [N*D]
   double* E;              // [N]
@@ -123,7 +136,6 @@ struct DpPairArgs {
   const double* xi;       // injected normals [N*D] in original order, or null
   u64 seed;
   DpCtl* ctl;
-  u64* dispmax;           // [R] bit patterns of max displacement^2
   i32 wrap;               // 1: wrap positions into the box after the update
   i32 pad1_;
 };
@@ -221,15 +233,31 @@ template <> struct DpPow3<0> { static constexpr int value = 1; };
 template <int D>
 struct DpTileSmem {
   i32 hstart[DP_HALO_MAX];     // first slot of each halo cell
+  i32 hcoord[DP_HALO_MAX];     // halo-box coordinates of each halo cell, 8 bits per dimension
   i32 hbase[DP_HALO_MAX + 1];  // staged-record offset of each halo cell (prefix)
   i32 home_u[DP_HOME_MAX];     // halo index of each home cell (-1: outside the box)
   i32 home_p[DP_HOME_MAX + 1]; // home-particle prefix over home cells
   i32 st_off[DpPow3<D>::value];// halo-index offset of each stencil entry
   i32 nrec;                    // staged records in the halo
+  i32 maxc;                    // largest halo-cell occupancy
   i32 nhomep;                  // home particles
   i32 rep;                     // replica of the tile
   i32 fits;                    // 1 if the halo fits in the LDS record area
 };
+
+// neighbor_list.py:183-201 trigger for one particle: squared displacement
+// beyond (skin / 2)^2; NaN also triggers, so that a rebuild reports it.
+__device__ __forceinline__ bool dp_skin_trips(double s, double skin_half_sq) {
+  return s > skin_half_sq || s != s;
+}
+
+// Raise ctl->need_rebuild if any lane of the wavefront tripped: a plain store
+// of 1 by one lane, only from wavefronts that tripped (no same-address atomics).
+__device__ __forceinline__ void dp_raise_rebuild(bool trip, DpCtl* ctl) {
+  const u64 b = __ballot(trip);
+  if (b != 0ull && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)b) - 1)
+    *(volatile i32*)&ctl->need_rebuild = 1;
+}
 
 // Replica owning global tile t.
 __device__ __forceinline__ int dp_tile_replica(const DpRep* reps, int R, i64 t) {
@@ -306,6 +334,10 @@ __device__ void dp_tile_setup(DpTileSmem<D>& S, const DpRep* reps, int R, const 
     const i64 cell = P.c0 + dp_linear<D>(cc, P.nc, P.order);
     const i32 s0 = cstart[cell];
     S.hstart[u] = s0;
+    i32 pk = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) pk |= hc[d] << (8 * d);
+    S.hcoord[u] = pk;
     S.hbase[u + 1] = cstart[cell + 1] - s0;
   }
   const int nhm = P.nhome;
@@ -321,6 +353,15 @@ __device__ void dp_tile_setup(DpTileSmem<D>& S, const DpRep* reps, int R, const 
       if (!coll && tc[d] * P.T[d] + mc[d] >= P.nc[d]) valid = false;
     }
     S.home_u[m] = valid ? (int)dp_linear<D>(hc, P.Th, P.order) : -1;
+    int cnt = 0;
+    if (valid) {
+      int cc[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) cc[d] = (P.nc[d] == 1) ? 0 : tc[d] * P.T[d] + mc[d];
+      const i64 cell = P.c0 + dp_linear<D>(cc, P.nc, P.order);
+      cnt = cstart[cell + 1] - cstart[cell];
+    }
+    S.home_p[m + 1] = cnt;
   }
   constexpr int n3 = DpPow3<D>::value;
   for (int st = tid; st < n3; st += blockDim.x) {
@@ -343,20 +384,119 @@ __device__ void dp_tile_setup(DpTileSmem<D>& S, const DpRep* reps, int R, const 
     S.st_off[st] = off;
   }
   __syncthreads();
-  if (tid == 0) {
-    S.hbase[0] = 0;
-    for (int u = 0; u < nh; ++u) S.hbase[u + 1] += S.hbase[u];
-    S.nrec = S.hbase[nh];
-    S.home_p[0] = 0;
-    for (int m = 0; m < nhm; ++m) {
-      const int u = S.home_u[m];
-      S.home_p[m + 1] = S.home_p[m] + (u >= 0 ? S.hbase[u + 1] - S.hbase[u] : 0);
+  // exclusive prefixes: wavefront 0 over the halo cells, wavefront 1 over the
+  // home cells (in-register scans in chunks of 64, carry kept in a register)
+  const int lane = tid & 63;
+  if (tid < 64) {
+    int carry = 0, mx = 0;
+    for (int u0 = 0; u0 < nh; u0 += 64) {
+      const int u = u0 + lane;
+      const int v = (u < nh) ? S.hbase[u + 1] : 0;
+      int incl = v;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(incl, off);
+        if (lane >= off) incl += t;
+      }
+      if (u < nh) S.hbase[u + 1] = carry + incl;
+      mx = max(mx, v);
+      carry += __shfl(incl, 63);
     }
-    S.nhomep = S.home_p[nhm];
-    S.rep = r;
-    S.fits = S.nrec <= cap ? 1 : 0;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
+    if (lane == 0) {
+      S.hbase[0] = 0;
+      S.nrec = carry;
+      S.maxc = mx;
+      S.rep = r;
+      S.fits = carry <= cap ? 1 : 0;
+    }
+  } else if (tid < 128) {
+    int carry = 0;
+    for (int m0 = 0; m0 < nhm; m0 += 64) {
+      const int m = m0 + lane;
+      int incl = (m < nhm) ? S.home_p[m + 1] : 0;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(incl, off);
+        if (lane >= off) incl += t;
+      }
+      if (m < nhm) S.home_p[m + 1] = carry + incl;
+      carry += __shfl(incl, 63);
+    }
+    if (lane == 0) {
+      S.home_p[0] = 0;
+      S.nhomep = carry;
+    }
   }
   __syncthreads();
+}
+
+// Tile tables in global memory, written by the neighbour-list emit at every
+// build (cell contents are fixed between builds) and loaded by the per-step
+// kernels with one coalesced copy instead of recomputing them:
+//   [0] nrec [1] nhomep [2] rep [3] maxc [4] nhalo [5] nhome [6..7] unused,
+//   then hstart[nh], hbase[nh + 1], hcoord[nh], home_u[nhm], home_p[nhm + 1].
+#define DP_TT_HEAD 8
+__host__ __device__ __forceinline__ int dp_tile_table_ints(int nh, int nhm) {
+  return DP_TT_HEAD + 3 * nh + 1 + 2 * nhm + 1;
+}
+
+template <int D>
+__device__ void dp_tile_store(const DpTileSmem<D>& S, const DpRep& P, i32* tt) {
+  const int nh = P.nhalo, nhm = P.nhome;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    tt[0] = S.nrec;
+    tt[1] = S.nhomep;
+    tt[2] = S.rep;
+    tt[3] = S.maxc;
+    tt[4] = nh;
+    tt[5] = nhm;
+  }
+  i32* p = tt + DP_TT_HEAD;
+  for (int u = tid; u < nh; u += blockDim.x) {
+    p[u] = S.hstart[u];
+    p[2 * nh + 1 + u] = S.hcoord[u];
+  }
+  for (int u = tid; u <= nh; u += blockDim.x) p[nh + u] = S.hbase[u];
+  for (int m = tid; m < nhm; m += blockDim.x) p[3 * nh + 1 + m] = S.home_u[m];
+  for (int m = tid; m <= nhm; m += blockDim.x) p[3 * nh + 1 + nhm + m] = S.home_p[m];
+}
+
+// Load tile t's tables (every thread calls this; it ends with a barrier).
+template <int D>
+__device__ void dp_tile_load(DpTileSmem<D>& S, const i32* tt, int cap) {
+  const int tid = threadIdx.x;
+  const int4 h0 = *(const int4*)tt;
+  const int4 h1 = *(const int4*)(tt + 4);
+  const int nh = h1.x, nhm = h1.y;
+  const i32* p = tt + DP_TT_HEAD;
+  for (int u = tid; u < nh; u += blockDim.x) {
+    S.hstart[u] = p[u];
+    S.hcoord[u] = p[2 * nh + 1 + u];
+  }
+  for (int u = tid; u <= nh; u += blockDim.x) S.hbase[u] = p[nh + u];
+  for (int m = tid; m < nhm; m += blockDim.x) S.home_u[m] = p[3 * nh + 1 + m];
+  for (int m = tid; m <= nhm; m += blockDim.x) S.home_p[m] = p[3 * nh + 1 + nhm + m];
+  if (tid == 0) {
+    S.nrec = h0.x;
+    S.nhomep = h0.y;
+    S.rep = h0.z;
+    S.maxc = h0.w;
+    S.fits = h0.x <= cap ? 1 : 0;
+  }
+  __syncthreads();
+}
+
+// Fixed-point offset of halo cell u's centre from the tile's first home cell
+// centre along d: (halo coordinate - 1) * Wq (0 along a collapsed dimension,
+// where the single cell's centre is the common origin).  Modulo 2^32: only
+// differences of shifted coordinates are used, and those are exact.
+template <int D>
+__device__ __forceinline__ u32 dp_halo_shift(const DpTileSmem<D>& S, const DpRep& P, int u, int d) {
+  if (P.nc[d] == 1) return 0u;
+  return (u32)(((S.hcoord[u] >> (8 * d)) & 255) - 1) * (u32)P.Wq[d];
 }
 
 // Halo cell holding staged record idx (binary search over hbase).

@@ -28,32 +28,29 @@ __global__ void k_fill_u64(u64* p, i64 n, u64 v) {
 }
 
 // ---------------------------------------------------------------------------
-// Control: decide whether this step rebuilds (neighbor_list.py:183-201 on the
-// maximum computed by the previous integrator pass), reset the maximum, and
-// take the RNG counter for this step.
+// Control: decide whether this step rebuilds (neighbor_list.py:183-201: the
+// previous integrator pass raised need_rebuild iff some particle's squared
+// displacement exceeds (skin / 2)^2, or is NaN), clear the flag, and take the
+// RNG counter for this step.  Any replica's trigger rebuilds all replicas.
 // ---------------------------------------------------------------------------
-__global__ void k_prologue(DpCtl* ctl, u64* dispmax, const DpRep* reps, int R, i64* nl_updates,
-                           int advance_step) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  int old = 0;
-  for (int r = 0; r < R; ++r) {
-    const double m = __longlong_as_double((long long)dispmax[r]);
-    if (m > reps[r].skin_half_sq) old = 1;
-  }
+__global__ void k_prologue(DpCtl* ctl, const DpRep* reps, int R, i64* nl_updates, int advance_step) {
+  if (blockIdx.x != 0) return;
+  const int tid = threadIdx.x;
+  const int old = ctl->need_rebuild;
   const int need = old | ctl->force_rebuild;
+  __syncthreads();
+  if (need) {
+    for (int k = tid; k < DP_STAT_SLOTS * DP_STAT_STRIDE; k += blockDim.x) ctl->slot[k] = 0;
+  }
+  if (tid != 0) return;
   ctl->rebuild = need;
   ctl->force_rebuild = 0;
-  if (need) {
-    ctl->max_count = 0;
-    ctl->max_cell = 0;
-    ctl->max_halo = 0;
-  }
+  ctl->need_rebuild = 0;
   // only skin-triggered rebuilds are counted here; requested ones are counted
   // by the caller that requested them
   if (old) {
     for (int r = 0; r < R; ++r) nl_updates[r] += 1;
   }
-  for (int r = 0; r < R; ++r) dispmax[r] = 0ull;
   if (advance_step) {
     ctl->step_index = ctl->step_counter;
     ctl->step_counter += 1;
@@ -67,6 +64,10 @@ __global__ void k_advance_step(DpCtl* ctl) {
   }
 }
 
+__global__ void k_clear_need(DpCtl* ctl) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->need_rebuild = 0;
+}
+
 __global__ void k_request_rebuild(DpCtl* ctl) {
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl->force_rebuild = 1;
 }
@@ -75,7 +76,8 @@ __global__ void k_request_rebuild(DpCtl* ctl) {
 // after the host uploaded new positions).
 template <int D>
 __global__ void k_skin_check(const double* x, const double* xref, const i32* rep, const DpRep* reps, i64 N,
-                             u64* dispmax) {
+                             DpCtl* ctl) {
+  bool trip = false;
   DP_FOR(i, N) {
     const DpRep& R = reps[rep[i]];
     double s = 0.0;
@@ -85,9 +87,9 @@ __global__ void k_skin_check(const double* x, const double* xref, const i32* rep
       t = dp_minimage_gt(t, R.L[d], R.halfL[d]);
       s = (d == 0) ? dp_mul(t, t) : dp_add(s, dp_mul(t, t));
     }
-    if (s != s) s = __longlong_as_double(0x7ff0000000000000ll);  // NaN -> +inf: forces a rebuild
-    atomicMax(&dispmax[rep[i]], (unsigned long long)__double_as_longlong(s));
+    trip |= dp_skin_trips(s, R.skin_half_sq);
   }
+  dp_raise_rebuild(trip, ctl);
 }
 
 // ---------------------------------------------------------------------------
@@ -321,89 +323,72 @@ __global__ void k_make_records(i64 N, const double* x, const i32* type, const i3
 
 // ---------------------------------------------------------------------------
 // Neighbour-list emit, one workgroup per tile.  The halo box's FP64 positions
-// are staged in LDS; each wavefront takes one home particle at a time and
-// sweeps its stencil rows (3 consecutive halo cells along the fastest digit)
-// 64 candidates at a time.  A candidate is kept iff the FP64 distance test of
-// neighbor_list.py:24-33 holds (evaluated as s < threshold, which is exactly
-// sqrt(s) < cutoff); ballot compaction keeps the stencil order of
-// neighbor_list.py:150-157 (order 0) / :82-86 (order 1) and ascending
-// original id inside a cell.  Entries are (stencil index << ob) | offset of
-// the neighbour inside its cell.
+// at the last build (xref, slot order) are staged in LDS; each wavefront takes
+// one home particle at a time and sweeps its stencil rows (3 consecutive halo
+// cells along the fastest digit) 64 candidates at a time.  A candidate is kept
+// iff the FP64 distance test of neighbor_list.py:24-33 holds (evaluated as
+// s < threshold, which is exactly sqrt(s) < cutoff); ballot compaction keeps
+// the stencil order of neighbor_list.py:150-157 (order 0) / :82-86 (order 1)
+// and ascending original id inside a cell.  An entry is the candidate's index
+// in the tile's staged halo: cell occupancies are fixed between builds, so the
+// pair kernels read a neighbour's record straight from LDS with it.  Reading
+// xref makes the emit repeatable: a new tile plan re-emits the same list.
 // ---------------------------------------------------------------------------
-template <int D, typename E>
-__global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile(const double* x, const i32* cstart,
-                                                               const DpRep* reps, int R, i64 ntiles, int K,
-                                                               int cap, int ob, E* nl, i32* nlcnt, DpCtl* ctl) {
-  if (!ctl->rebuild) return;
+template <int D, typename E, bool LDS>
+__device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const DpRep& P, const double* x,
+                                                  const double* sx, int K, E* nl, i32* nlcnt, DpCtl* ctl,
+                                                  int& st_count) {
   constexpr int N3 = DpPow3<D>::value;
   constexpr int NROW = N3 / 3;  // <= 27 for D <= 4: one row per lane
-  __shared__ DpTileSmem<D> S;
-  extern __shared__ double sx[];  // cap * D staged positions
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int nwaves = blockDim.x >> 6;
-  const u32 offmax = (1u << ob) - 1u;
-  for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    dp_tile_setup<D>(S, reps, R, cstart, t, cap);
-    const DpRep& P = reps[S.rep];
-    const int nh = P.nhalo;
-    const bool fits = S.fits != 0;
-    const int fastest = (P.order == 0) ? 0 : D - 1;
-    const int dx0 = (P.nc[fastest] == 1) ? 1 : 0;  // only s = 0 along a collapsed fastest dimension
-    if (fits) {
-      for (int idx = tid; idx < S.nrec; idx += blockDim.x) {
-        const int u = dp_halo_of<D>(S, nh, idx);
-        const i64 slot = S.hstart[u] + (idx - S.hbase[u]);
+  const i64 Ks = (i64)((K + 7) & ~7);  // row stride: rows are read 8 bytes at a time
+  // Every particle of a home cell has the same candidates (its stencil rows,
+  // in order).  Wavefront w takes particles w, w + nwaves, ... of each home
+  // cell; for each chunk of 64 candidates it finds their staged indices and
+  // positions once, then tests each of its particles against the whole chunk
+  // (particle position: LDS broadcast; running row length: one lane each).
+  for (int m = 0; m < P.nhome; ++m) {
+    const int uh = S.home_u[m];
+    if (uh < 0) continue;
+    const int self0 = S.hbase[uh];
+    const int np = S.hbase[uh + 1] - self0;
+    const i64 slot0 = S.hstart[uh];
+    // lane r < NROW holds stencil row r: its valid cells are consecutive halo cells
+    int r_lo = 0, r_len = 0, r_u0 = 0;
+    if (lane < NROW) {
+      bool any = false;
+      int hi = 0;
+      for (int dx = 0; dx < 3; ++dx) {
+        const int st = 3 * lane + dx;
+        int sd[D];
+        dp_stencil_digits<D>(st, P.order, sd);
+        bool ok = true;
 #pragma unroll
-        for (int d = 0; d < D; ++d) sx[idx * D + d] = x[slot * D + d];
+        for (int d = 0; d < D; ++d)
+          if (P.nc[d] == 1 && sd[d] != 0) ok = false;
+        if (!ok) continue;
+        const int u = uh + S.st_off[st];
+        if (!any) { any = true; r_u0 = u; r_lo = S.hbase[u]; }
+        hi = S.hbase[u + 1];
       }
+      r_len = any ? hi - r_lo : 0;
     }
-    if (tid == 0) {
-      int mc = 0;
-      for (int u = 0; u < nh; ++u) mc = max(mc, S.hbase[u + 1] - S.hbase[u]);
-      if ((u32)mc > offmax + 1u) ctl->cell_overflow = 1;
-      atomicMax(&ctl->max_cell, mc);
-      atomicMax(&ctl->max_halo, S.nrec);
+    int incl = r_len;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int vv = __shfl_up(incl, off);
+      if (lane >= off) incl += vv;
     }
-    __syncthreads();
-    for (int k = wave; k < S.nhomep; k += nwaves) {
-      const int m = dp_home_of<D>(S, P.nhome, k);
-      const int uh = S.home_u[m];
-      const int self = S.hbase[uh] + (k - S.home_p[m]);
-      const i64 slot_i = S.hstart[uh] + (k - S.home_p[m]);
-      double xi[D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) xi[d] = x[slot_i * D + d];
-      // lane r < NROW holds stencil row r: its valid cells are consecutive halo cells
-      int r_lo = 0, r_len = 0, r_u0 = 0;
-      if (lane < NROW) {
-        bool any = false;
-        int hi = 0;
-        for (int dx = 0; dx < 3; ++dx) {
-          const int st = 3 * lane + dx;
-          int sd[D];
-          dp_stencil_digits<D>(st, P.order, sd);
-          bool ok = true;
-#pragma unroll
-          for (int d = 0; d < D; ++d)
-            if (P.nc[d] == 1 && sd[d] != 0) ok = false;
-          if (!ok) continue;
-          const int u = uh + S.st_off[st];
-          if (!any) { any = true; r_u0 = u; r_lo = S.hbase[u]; }
-          hi = S.hbase[u + 1];
-        }
-        r_len = any ? hi - r_lo : 0;
-      }
-      int incl = r_len;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int vv = __shfl_up(incl, off);
-        if (lane >= off) incl += vv;
-      }
-      const int r_pref = incl - r_len;
-      const int ncand = __shfl(incl, NROW - 1);
-      int cnt = 0;
+    const int r_pref = incl - r_len;
+    const int ncand = __shfl(incl, NROW - 1);
+    // this wavefront's particles: k = wave + nwaves * j, j = 0 .. npw - 1
+    const int npw = (np > wave) ? (np - wave + nwaves - 1) / nwaves : 0;
+    for (int j0 = 0; j0 < npw; j0 += 64) {
+      const int nj = min(64, npw - j0);
+      int cnt = 0;  // lane jj: row length of particle k = wave + nwaves * (j0 + jj)
       for (int base = 0; base < ncand; base += 64) {
         const int c = base + lane;
         // row of candidate c: largest r with r_pref[r] <= c (fixed-step search)
@@ -417,37 +402,112 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile(const double* x, 
         const int lo_row = __shfl(r_lo, row);
         const int pf_row = __shfl(r_pref, row);
         const int u0_row = __shfl(r_u0, row);
-        bool hit = false;
-        E ent = 0;
-        if (c < ncand) {
-          const int idx = lo_row + (c - pf_row);
-          int u = u0_row;
-          while (S.hbase[u + 1] <= idx) ++u;
-          const int st = 3 * row + dx0 + (u - u0_row);
-          const int off = idx - S.hbase[u];
+        const bool valid = c < ncand;
+        const int idx = valid ? lo_row + (c - pf_row) : 0;
+        double xj[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) xj[d] = 0.0;
+        if (valid) {
+          if (LDS) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) xj[d] = sx[idx * D + d];
+          } else {
+            int u = u0_row;
+            while (S.hbase[u + 1] <= idx) ++u;
+            const i64 slot_j = (i64)S.hstart[u] + (idx - S.hbase[u]);
+#pragma unroll
+            for (int d = 0; d < D; ++d) xj[d] = x[slot_j * D + d];
+          }
+        }
+        for (int jj = 0; jj < nj; ++jj) {
+          const int k = wave + nwaves * (j0 + jj);
+          const int self = self0 + k;
+          const i64 slot_i = slot0 + k;
           double ss = 0.0;
 #pragma unroll
           for (int d = 0; d < D; ++d) {
-            const double xj = fits ? sx[idx * D + d] : x[((i64)S.hstart[u] + off) * D + d];
-            const double tt = dp_minimage_lt(dp_sub(xi[d], xj), P.L[d], P.halfL[d]);
+            const double xi = LDS ? sx[self * D + d] : x[slot_i * D + d];
+            const double tt = dp_minimage_lt(dp_sub(xi, xj[d]), P.L[d], P.halfL[d]);
             ss = (d == 0) ? dp_mul(tt, tt) : dp_add(ss, dp_mul(tt, tt));
           }
-          hit = (idx != self) && (ss < P.nl_thresh);
-          ent = (E)(((u32)st << ob) | ((u32)off & offmax));
+          const bool hit = valid && (idx != self) && (ss < P.nl_thresh);
+          const u64 mask = __ballot(hit);
+          const int c0 = __builtin_amdgcn_readlane(cnt, jj);
+          const int pos = c0 + __popcll(mask & ((1ull << lane) - 1ull));
+          if (hit && pos < K) nl[slot_i * Ks + pos] = (E)idx;
+          if (lane == jj) cnt += __popcll(mask);
         }
-        const u64 mask = __ballot(hit);
-        const int pos = cnt + __popcll(mask & ((1ull << lane) - 1ull));
-        if (hit && pos < K) nl[slot_i * K + pos] = ent;
-        cnt += __popcll(mask);
       }
-      if (lane == 0) {
-        nlcnt[slot_i] = cnt;
+      if (lane < nj) {
+        const int k = wave + nwaves * (j0 + lane);
+        nlcnt[slot0 + k] = cnt;
         if (cnt > K) ctl->overflow = 1;
-        atomicMax(&ctl->max_count, cnt);
+      }
+      int mx = (lane < nj) ? cnt : 0;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
+      st_count = max(st_count, mx);
+    }
+  }
+}
+
+template <int D, typename E>
+__global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile(const double* xref, const i32* cstart,
+                                                               const DpRep* reps, int R, i64 ntiles, int K,
+                                                               int cap, E* nl, i32* nlcnt, i32* tiletab,
+                                                               int tstride, DpCtl* ctl, int force) {
+  if (!force && !ctl->rebuild) return;
+  // non-finite positions were binned into arbitrary cells: skip the (then
+  // unbounded) emit; the error is reported after the step block
+  if (ctl->nonfinite) return;
+  __shared__ DpTileSmem<D> S;
+  extern __shared__ __align__(16) double sx[];  // cap * D staged positions
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int nwaves = blockDim.x >> 6;
+  int st_count = 0, st_cell = 0, st_halo = 0;  // running maxima (per wavefront / thread 0)
+  for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    dp_tile_setup<D>(S, reps, R, cstart, t, cap);
+    const DpRep& P = reps[S.rep];
+    const bool fits = S.fits != 0;
+    dp_tile_store<D>(S, P, tiletab + t * (i64)tstride);
+    if (fits) {
+      for (int idx = tid; idx < S.nrec; idx += blockDim.x) {
+        const int u = dp_halo_of<D>(S, P.nhalo, idx);
+        const i64 slot = S.hstart[u] + (idx - S.hbase[u]);
+#pragma unroll
+        for (int d = 0; d < D; ++d) sx[idx * D + d] = xref[slot * D + d];
       }
     }
+    if (tid == 0) {
+      // entries index the staged halo: E must hold nrec - 1
+      if (sizeof(E) < 4 && S.nrec > 65536) ctl->cell_overflow = 1;
+      st_cell = max(st_cell, S.maxc);
+      st_halo = max(st_halo, S.nrec);
+    }
+    __syncthreads();
+    if (fits)
+      dp_emit_particles<D, E, true>(S, P, xref, sx, K, nl, nlcnt, ctl, st_count);
+    else
+      dp_emit_particles<D, E, false>(S, P, xref, sx, K, nl, nlcnt, ctl, st_count);
     __syncthreads();
   }
+  // statistics: one update per workgroup, spread over the slots
+  __shared__ int red[DP_TILE_THREADS / 64];
+  if (lane == 0) red[wave] = st_count;
+  __syncthreads();
+  if (tid == 0) {
+    for (int w = 1; w < nwaves; ++w) st_count = max(st_count, red[w]);
+    i32* sl = ctl->slot + (blockIdx.x % DP_STAT_SLOTS) * DP_STAT_STRIDE;
+    atomicMax(sl + DP_STAT_COUNT, st_count);
+    atomicMax(sl + DP_STAT_CELL, st_cell);
+    atomicMax(sl + DP_STAT_HALO, st_halo);
+  }
+}
+
+__global__ void k_clear_stats(DpCtl* ctl) {
+  for (int k = threadIdx.x; k < DP_STAT_SLOTS * DP_STAT_STRIDE; k += blockDim.x) ctl->slot[k] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -469,38 +529,34 @@ __global__ void k_from_orig(const T* src, T* dst, const i32* orig, i64 N, int W)
   }
 }
 
-// Neighbour list in the reference layout: row orig[i], entries orig[j],
-// -1 padded to K (neighbor_list.py:14-17).  Entries are decoded with the
-// particle's cell and the stencil.
+// Neighbour-list export in the reference layout (neighbor_list.py:14-17: row
+// = original id, -1 after the last neighbour): one workgroup per tile decodes the staged-halo indices
+// of its home particles' rows through the tile tables.
 template <int D, typename E>
-__global__ void k_nl_export(const E* nl, const i32* nlcnt, const i32* orig, const i32* cell_of, const i32* cstart,
-                            const i32* rep, const DpRep* reps, i64 N, int K, int ob, i32* out) {
-  DP_FOR(i, N) {
-    const DpRep& P = reps[rep[i]];
-    int ci[D];
-    dp_unlinear<D>((i64)cell_of[i] - P.c0, P.nc, P.order, ci);
-    const i64 row = orig[i];
-    const int cnt = nlcnt[i] < K ? nlcnt[i] : K;
-    for (int k = 0; k < K; ++k) {
-      i32 val = -1;
-      if (k < cnt) {
-        const u32 e = (u32)nl[i * K + k];
-        const int st = (int)(e >> ob);
-        const int off = (int)(e & ((1u << ob) - 1u));
-        int sd[D], cc[D];
-        dp_stencil_digits<D>(st, P.order, sd);
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          int c = ci[d] + (P.nc[d] == 1 ? 0 : sd[d]);
-          if (c < 0) c += P.nc[d];
-          if (c >= P.nc[d]) c -= P.nc[d];
-          cc[d] = c;
+__global__ void __launch_bounds__(DP_TILE_THREADS) k_nl_export(const E* nl, const i32* nlcnt, const i32* orig,
+                                                               const i32* cstart, const DpRep* reps, int R,
+                                                               i64 ntiles, int K, i32* out) {
+  __shared__ DpTileSmem<D> S;
+  for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    dp_tile_setup<D>(S, reps, R, cstart, t, 0x7fffffff);
+    const DpRep& P = reps[S.rep];
+    for (int k = threadIdx.x; k < S.nhomep; k += blockDim.x) {
+      const int m = dp_home_of<D>(S, P.nhome, k);
+      const int uh = S.home_u[m];
+      const i64 slot_i = S.hstart[uh] + (k - S.home_p[m]);
+      const i64 row = orig[slot_i];
+      const int cnt = nlcnt[slot_i] < K ? nlcnt[slot_i] : K;
+      for (int kk = 0; kk < K; ++kk) {
+        i32 val = -1;
+        if (kk < cnt) {
+          const int idx = (int)nl[slot_i * (i64)((K + 7) & ~7) + kk];
+          const int u = dp_halo_of<D>(S, P.nhalo, idx);
+          val = orig[S.hstart[u] + (idx - S.hbase[u])];
         }
-        const i64 cell = P.c0 + dp_linear<D>(cc, P.nc, P.order);
-        val = orig[cstart[cell] + off];
+        out[row * K + kk] = val;
       }
-      out[row * K + k] = val;
     }
+    __syncthreads();
   }
 }
 

@@ -81,6 +81,7 @@ CodeCache& code_cache() {
 
 struct PairConfig {
   int D, mode, ntypes, even, rs, tpp, nlw, ob, pairprec;
+  int cmask;  // dimensions with a single cell in some replica
 };
 
 // Stencil-index bits of a neighbour entry (3^D stencil cells).
@@ -97,8 +98,8 @@ std::string pair_source(const PairConfig& c, const char* device_src) {
   char buf[512];
   snprintf(buf, sizeof(buf),
            "#define DP_D %d\n#define DP_MODE %d\n#define DP_NTYPES %d\n#define DP_EVEN %d\n#define DP_RS %d\n"
-           "#define DP_TPP %d\n#define DP_NLW %d\n#define DP_OB %d\n#define DP_PAIRPREC %d\n",
-           c.D, c.mode, c.ntypes, c.even ? 1 : 0, c.rs, c.tpp, c.nlw, c.ob, c.pairprec);
+           "#define DP_TPP %d\n#define DP_NLW %d\n#define DP_OB %d\n#define DP_PAIRPREC %d\n#define DP_CMASK %d\n",
+           c.D, c.mode, c.ntypes, c.even ? 1 : 0, c.rs, c.tpp, c.nlw, c.ob, c.pairprec, c.cmask);
   src += buf;
   src += "#include \"dp_device.h\"\n";
   src += device_src;
@@ -173,7 +174,6 @@ struct dp_ctx {
   DpPairTab* tab_d = nullptr;
   DpCtl* ctl_d = nullptr;
   DpCtl* ctl_h = nullptr;  // pinned
-  u64* dispmax_d = nullptr;
   i64* nlupd_d = nullptr;
   double* x[2] = {nullptr, nullptr};
   i32* xq[2] = {nullptr, nullptr};
@@ -188,6 +188,7 @@ struct dp_ctx {
   int nlw = 16, ob = 11, tpp = 8, qbits = 24, cap = 1024, cap_emit = 1024;
   // tuning (0 = automatic)
   int tune_tpp = 0, tune_tile = 0, fused = 1;
+  int cmask = 0;  // OR of the replicas' collapsed-dimension masks
   i64 ntiles = 0;
   int tile_grid = 8;
   size_t lds_pair = 0, lds_emit = 0;
@@ -206,6 +207,11 @@ struct dp_ctx {
 
   // state flags
   bool nl_built = false;
+  i32* tiletab = nullptr;  // per-tile tables written at every build
+  int tstride = 0;
+  size_t tiletab_bytes = 0;
+  bool nl_valid = true;                 // false when the list buffer was reallocated
+  std::vector<i64> cells_sig, tiles_sig;  // geometry the list was built for
   bool pos_dirty = true;
   u64 seed = 0x5eed0d0bull;
 
@@ -287,10 +293,14 @@ double ball_volume(int D) {
 //  - lanes per particle (tpp) from the expected list length;
 //  - tiles: enough home cells that one pass of the workgroup's lane groups
 //    is about full;
-//  - fixed-point scale: offsets from a cell centre and pair displacements
-//    stay below 2^30 units;
-//  - entries: 16-bit (stencil index, offset in cell) when cells are small
-//    enough, else 32-bit.
+//  - fixed-point scale: coordinates relative to a tile's first home cell
+//    centre stay below 2^30 units;
+//  - entries: index of the neighbour in its tile's staged halo, 16-bit when
+//    halos are small enough, else 32-bit.
+// A new plan over the same cells re-emits the existing list (from xref)
+// instead of rebuilding it, so rebuild times do not depend on the plan.
+int launch_records(dp_ctx* ctx);
+int launch_reemit(dp_ctx* ctx);
 int refresh_reps(dp_ctx* ctx) {
   if (!ctx->reps_dirty) return DP_OK;
   if (!ctx->have_box) return ctx->fail(DP_ERR_STATE, "dp_set_replicas has not been called");
@@ -383,11 +393,18 @@ int refresh_reps(dp_ctx* ctx) {
       P.halfL[d] = L / 2;
       P.w[d] = L / (double)nc[d];
       P.inv_w[d] = (double)nc[d] / L;
+      // fixed-point range: records hold offsets from their cell centre
+      // (|q| <= w/2 + skin between builds); halo-box coordinates and pair
+      // displacements are formed in wrapping 32-bit arithmetic, exact while
+      // the displacement itself fits (listed pairs: |dq| <= cutoff + 2 skin);
+      // a collapsed dimension uses offsets from the box centre
+      const double sk = std::max(ctx->skin[r], 0.0);
       if (nc[d] == 1) {
         P.collapsed_mask |= (1 << d);
         rmax = std::max(rmax, L);
       } else {
-        rmax = std::max(rmax, 0.5 * P.w[d] + ctx->skin[r] + (std::isfinite(cut) ? cut : 0.0));
+        rmax = std::max(rmax, P.w[d] + sk);  // also keeps Wq within 2^30
+        rmax = std::max(rmax, (std::isfinite(cut) ? cut : 0.0) + 2.0 * sk);
       }
     }
     P.c0 = c0;
@@ -407,6 +424,8 @@ int refresh_reps(dp_ctx* ctx) {
     P.tau = ctx->have_integ ? ctx->tau[r] : 0.1;
   }
   if (c0 >= (i64)0x7fffffff) return ctx->fail(DP_ERR_ARG, "too many cells (%lld)", (long long)c0);
+  ctx->cmask = 0;
+  for (const auto& P : reps) ctx->cmask |= P.collapsed_mask;
   // fixed-point scale: |values| < 2^30 units
   int qbits = (int)std::floor(30.0 - std::log2(rmax));
   if (qbits > 30) qbits = 30;
@@ -417,19 +436,32 @@ int refresh_reps(dp_ctx* ctx) {
       P.Wq[d] = (i32)std::rint(P.w[d] * qs);
       P.Lq[d] = (i32)std::min(std::rint(P.L[d] * qs), 2147483647.0);
     }
-  // entry format
-  const int sbits = stencil_bits(D);
-  int nlw = 16, ob = 16 - sbits;
-  if (occ_max * 4.0 + 16.0 >= (double)(1 << ob)) {
-    nlw = 32;
-    ob = 32 - sbits;
-  }
+  // entry format: the index of a neighbour in its tile's staged halo; 16
+  // bits unless a halo could hold more than 2^16 particles
+  (void)occ_max;
+  int nlw = 16;
+  if (halo_occ_max * 4.0 + 64.0 >= 65536.0) nlw = 32;
+  const int ob = nlw;
   // LDS records: 1.5x the mean halo, at most 48 KB
   const size_t rec_bytes = (ctx->mode == DP_MIXED) ? (size_t)ctx->rs * 4 : (size_t)D * 8 + 4;
   i64 cap = (i64)std::ceil(1.5 * halo_occ_max) + 64;
   cap = std::min<i64>(cap, (i64)(48 * 1024 / rec_bytes));
   cap = std::max<i64>(cap, 64);
-  const int old_nlw = ctx->nlw, old_tpp = ctx->tpp, old_ob = ctx->ob;
+  // geometry of the built list: cells (a change needs a rebuild) and tiles /
+  // entry width / fixed-point scale (a change needs a re-emit from xref)
+  std::vector<i64> cells_sig, tiles_sig;
+  for (const auto& P : reps) {
+    cells_sig.push_back(P.c0);
+    for (int d = 0; d < D; ++d) cells_sig.push_back(P.nc[d]);
+    tiles_sig.push_back(P.t0);
+    for (int d = 0; d < D; ++d) tiles_sig.push_back(P.T[d]);
+  }
+  tiles_sig.push_back(nlw);
+  tiles_sig.push_back(qbits);
+  const bool cells_changed = cells_sig != ctx->cells_sig;
+  const bool tiles_changed = tiles_sig != ctx->tiles_sig;
+  ctx->cells_sig = cells_sig;
+  ctx->tiles_sig = tiles_sig;
   ctx->nlw = nlw;
   ctx->ob = ob;
   ctx->tpp = tpp;
@@ -456,7 +488,27 @@ int refresh_reps(dp_ctx* ctx) {
     CHK(dalloc(ctx, ctx->tiles, (size_t)ctx->ntiles_scan + 1));
     release_graph(ctx);
   }
-  const size_t nl_bytes = (size_t)ctx->N * (size_t)ctx->K * (size_t)(nlw / 8);
+  // tile tables: one fixed stride for all tiles
+  {
+    int ts = 0;
+    for (const auto& P : reps) ts = std::max(ts, dp_tile_table_ints(P.nhalo, P.nhome));
+    ts = (ts + 31) & ~31;
+    const size_t bytes = (size_t)t0 * (size_t)ts * 4;
+    if (ts != ctx->tstride || bytes != ctx->tiletab_bytes || !ctx->tiletab) {
+      dfree(ctx->tiletab);
+      ctx->tiletab_bytes = 0;
+      if (hipMalloc(&ctx->tiletab, bytes) != hipSuccess) {
+        ctx->tiletab = nullptr;
+        return ctx->fail(DP_ERR_HIP, "cannot allocate tile tables of %zu bytes", bytes);
+      }
+      ctx->tiletab_bytes = bytes;
+      ctx->tstride = ts;
+      ctx->nl_valid = false;  // tables are written by the emit
+      release_graph(ctx);
+    }
+  }
+  // rows padded to a multiple of 8 entries (the pair kernels read 8 bytes at a time)
+  const size_t nl_bytes = (size_t)ctx->N * (size_t)((ctx->K + 7) & ~7) * (size_t)(nlw / 8);
   if (nl_bytes != ctx->nl_bytes || !ctx->nl) {
     dfree(ctx->nl);
     ctx->nl_bytes = 0;
@@ -465,17 +517,22 @@ int refresh_reps(dp_ctx* ctx) {
       return ctx->fail(DP_ERR_HIP, "cannot allocate neighbour list of %zu bytes", nl_bytes);
     }
     ctx->nl_bytes = nl_bytes;
-    ctx->nl_built = false;
+    ctx->nl_valid = false;  // re-emitted below when a list exists
     release_graph(ctx);
   }
-  if (old_nlw != nlw || old_tpp != tpp || old_ob != ob) {
-    ctx->nl_built = false;
-    release_graph(ctx);
-  }
+  if (cells_changed || tiles_changed) release_graph(ctx);
+  if (cells_changed) ctx->nl_built = false;
+  const bool reemit = ctx->nl_built && (tiles_changed || !ctx->nl_valid);
   HIPCHK(hipMemcpyAsync(ctx->reps_d, ctx->reps_h.data(), sizeof(DpRep) * ctx->R, hipMemcpyHostToDevice, ctx->st));
-  HIPCHK(hipStreamSynchronize(ctx->st));  // reps_h may change on the next call
   ctx->reps_dirty = false;
   ctx->tab_dirty = true;  // guard bands depend on the fixed-point scale
+  if (reemit) {
+    // same cells and list, new tile plan: records and entries follow the plan
+    CHK(launch_records(ctx));
+    CHK(launch_reemit(ctx));
+  }
+  ctx->nl_valid = true;
+  HIPCHK(hipStreamSynchronize(ctx->st));  // reps_h may change on the next call
   return DP_OK;
 }
 
@@ -535,7 +592,8 @@ int refresh_tab(dp_ctx* ctx) {
 // Compile (or fetch) the pair module for the current plan.
 int ensure_module(dp_ctx* ctx) {
   if (!ctx->have_pot) return ctx->fail(DP_ERR_STATE, "no potential loaded (dp_load_potential)");
-  PairConfig c{ctx->D, ctx->mode, ctx->ntypes, ctx->pot_even, ctx->rs, ctx->tpp, ctx->nlw, ctx->ob, ctx->pair_prec};
+  PairConfig c{ctx->D, ctx->mode, ctx->ntypes, ctx->pot_even, ctx->rs, ctx->tpp, ctx->nlw, ctx->ob, ctx->pair_prec,
+               ctx->cmask};
   std::string src = pair_source(c, ctx->pot_src.c_str());
   if (ctx->mod && src == ctx->mod_key) return DP_OK;
   std::vector<char> code;
@@ -580,6 +638,8 @@ DpPairArgs pair_args(dp_ctx* ctx, int cur) {
   a.K = ctx->K;
   a.R = ctx->R;
   a.cap = ctx->cap;
+  a.tiletab = ctx->tiletab;
+  a.tstride = ctx->tstride;
   a.x_out = ctx->x[1 - cur];
   a.xq_out = ctx->xq[1 - cur];
   a.v = ctx->v;
@@ -593,7 +653,6 @@ DpPairArgs pair_args(dp_ctx* ctx, int cur) {
   a.xi = nullptr;
   a.seed = ctx->seed;
   a.ctl = ctx->ctl_d;
-  a.dispmax = ctx->dispmax_d;
   a.wrap = 1;
   return a;
 }
@@ -612,6 +671,33 @@ int launch_flat(dp_ctx* ctx, hipFunction_t f, DpPairArgs& a) {
   void* params[] = {&a};
   HIPCHK(hipModuleLaunchKernel(f, grid, 1, 1, 256, 1, 1, 0, ctx->st, params, nullptr));
   return DP_OK;
+}
+
+// Neighbour-list emit over the current cells from xref; force = 1 runs it
+// outside a rebuild (same list, new tile plan).
+template <int D>
+int launch_emit_t(dp_ctx* ctx, int force) {
+  if (ctx->nlw == 16)
+    k_emit_tile<D, u16><<<ctx->tile_grid, DP_TILE_THREADS, ctx->lds_emit, ctx->st>>>(
+        ctx->xref, ctx->cstart, ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, ctx->cap_emit, (u16*)ctx->nl, ctx->nlcnt,
+        ctx->tiletab, ctx->tstride, ctx->ctl_d, force);
+  else
+    k_emit_tile<D, u32><<<ctx->tile_grid, DP_TILE_THREADS, ctx->lds_emit, ctx->st>>>(
+        ctx->xref, ctx->cstart, ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, ctx->cap_emit, (u32*)ctx->nl, ctx->nlcnt,
+        ctx->tiletab, ctx->tstride, ctx->ctl_d, force);
+  HIPCHK(hipGetLastError());
+  return DP_OK;
+}
+
+int launch_reemit(dp_ctx* ctx) {
+  k_clear_stats<<<1, 256, 0, ctx->st>>>(ctx->ctl_d);
+  switch (ctx->D) {
+    case 1: return launch_emit_t<1>(ctx, 1);
+    case 2: return launch_emit_t<2>(ctx, 1);
+    case 3: return launch_emit_t<3>(ctx, 1);
+    case 4: return launch_emit_t<4>(ctx, 1);
+  }
+  return ctx->fail(DP_ERR_ARG, "dimension %d unsupported", ctx->D);
 }
 
 template <int D>
@@ -636,15 +722,7 @@ int launch_rebuild_t(dp_ctx* ctx) {
                                                  ctx->orig, ctx->cell_of, ctx->scratch, ctx->xref, ctx->cellpack,
                                                  ctx->xq[cur], ctx->rs, std::ldexp(1.0, ctx->qbits), ctx->rep,
                                                  ctx->reps_d, ctx->ctl_d);
-  if (ctx->nlw == 16)
-    k_emit_tile<D, u16><<<ctx->tile_grid, DP_TILE_THREADS, ctx->lds_emit, st>>>(
-        ctx->x[cur], ctx->cstart, ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, ctx->cap_emit, ctx->ob, (u16*)ctx->nl,
-        ctx->nlcnt, ctx->ctl_d);
-  else
-    k_emit_tile<D, u32><<<ctx->tile_grid, DP_TILE_THREADS, ctx->lds_emit, st>>>(
-        ctx->x[cur], ctx->cstart, ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, ctx->cap_emit, ctx->ob, (u32*)ctx->nl,
-        ctx->nlcnt, ctx->ctl_d);
-  HIPCHK(hipGetLastError());
+  CHK(launch_emit_t<D>(ctx, 0));
   return DP_OK;
 }
 
@@ -660,9 +738,10 @@ int launch_rebuild(dp_ctx* ctx) {
 
 template <int D>
 int launch_skin_check_t(dp_ctx* ctx) {
-  k_fill_u64<<<1, DP_BLOCK, 0, ctx->st>>>(ctx->dispmax_d, ctx->R, 0ull);
+  // the flag of the last integrator pass refers to positions replaced since
+  k_clear_need<<<1, 64, 0, ctx->st>>>(ctx->ctl_d);
   k_skin_check<D><<<grid_for(ctx->N), DP_BLOCK, 0, ctx->st>>>(ctx->x[ctx->cur], ctx->xref, ctx->rep, ctx->reps_d,
-                                                              ctx->N, ctx->dispmax_d);
+                                                              ctx->N, ctx->ctl_d);
   HIPCHK(hipGetLastError());
   return DP_OK;
 }
@@ -697,8 +776,20 @@ int launch_records(dp_ctx* ctx) {
   return ctx->fail(DP_ERR_ARG, "dimension %d unsupported", ctx->D);
 }
 
+// Fold the per-slot neighbour-list statistics into max_count / max_cell /
+// max_halo of a host copy of the control block.
+void reduce_ctl(DpCtl* c) {
+  c->max_count = c->max_cell = c->max_halo = 0;
+  for (int k = 0; k < DP_STAT_SLOTS; ++k) {
+    const i32* sl = c->slot + k * DP_STAT_STRIDE;
+    c->max_count = std::max(c->max_count, sl[DP_STAT_COUNT]);
+    c->max_cell = std::max(c->max_cell, sl[DP_STAT_CELL]);
+    c->max_halo = std::max(c->max_halo, sl[DP_STAT_HALO]);
+  }
+}
+
 int launch_prologue(dp_ctx* ctx, int advance) {
-  k_prologue<<<1, 64, 0, ctx->st>>>(ctx->ctl_d, ctx->dispmax_d, ctx->reps_d, ctx->R, ctx->nlupd_d, advance);
+  k_prologue<<<1, 256, 0, ctx->st>>>(ctx->ctl_d, ctx->reps_d, ctx->R, ctx->nlupd_d, advance);
   HIPCHK(hipGetLastError());
   return DP_OK;
 }
@@ -707,6 +798,7 @@ int launch_prologue(dp_ctx* ctx, int advance) {
 int check_ctl(dp_ctx* ctx) {
   HIPCHK(hipMemcpyAsync(ctx->ctl_h, ctx->ctl_d, sizeof(DpCtl), hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
+  reduce_ctl(ctx->ctl_h);
   if (ctx->ctl_h->overflow || ctx->ctl_h->nonfinite || ctx->ctl_h->cell_overflow) {
     const bool ov = ctx->ctl_h->overflow != 0;
     const bool co = ctx->ctl_h->cell_overflow != 0;
@@ -838,8 +930,8 @@ DP_API int dp_version(void) { return 100; }
 
 // Default plan used by the device-free compile checks.
 PairConfig default_config(int dim, int precision, int ntypes, int even, int pair_precision) {
-  PairConfig c{dim, precision, ntypes, even, rs_for_dim(dim), 8, 16, 16 - stencil_bits(dim),
-               pair_precision ? 32 : 64};
+  PairConfig c{dim, precision, ntypes, even, rs_for_dim(dim), 8, 16, 16,
+               pair_precision ? 32 : 64, 0};
   return c;
 }
 
@@ -848,6 +940,32 @@ DP_API int dp_compile_check(int dim, int precision, int ntypes, int even, const 
   if (dim < 1 || dim > DP_DMAX || !device_src) return DP_ERR_ARG;
   if (ntypes < 1 || ntypes > DP_MAXTYPES) return DP_ERR_ARG;
   std::string src = pair_source(default_config(dim, precision, ntypes, even, pair_precision), device_src);
+  std::vector<char> code;
+  std::string lg;
+  int rc = compile_cached(src, code, lg);
+  if (log && log_len > 0) {
+    std::strncpy(log, lg.c_str(), (size_t)log_len - 1);
+    log[log_len - 1] = '\0';
+  }
+  if (code_size) *code_size = (int64_t)code.size();
+  return rc;
+}
+
+DP_API int dp_compile_check_tuned(int dim, int precision, int ntypes, int even, const char* device_src,
+                                  int pair_precision, int threads_per_particle, int entry_bits,
+                                  int collapsed_mask, char* log, int64_t log_len, int64_t* code_size) {
+  if (dim < 1 || dim > DP_DMAX || !device_src) return DP_ERR_ARG;
+  if (ntypes < 1 || ntypes > DP_MAXTYPES) return DP_ERR_ARG;
+  const int t = threads_per_particle;
+  if (t != 1 && t != 2 && t != 4 && t != 8 && t != 16) return DP_ERR_ARG;
+  if (entry_bits != 16 && entry_bits != 32) return DP_ERR_ARG;
+  PairConfig c = default_config(dim, precision, ntypes, even, pair_precision);
+  if (collapsed_mask < 0 || collapsed_mask >= (1 << dim)) return DP_ERR_ARG;
+  c.tpp = t;
+  c.nlw = entry_bits;
+  c.ob = entry_bits;
+  c.cmask = collapsed_mask;
+  std::string src = pair_source(c, device_src);
   std::vector<char> code;
   std::string lg;
   int rc = compile_cached(src, code, lg);
@@ -932,7 +1050,6 @@ DP_API int dp_create(dp_ctx** out, int device, int dim, int64_t n_total, int n_r
   CCHK(dalloc(ctx, ctx->reps_d, (size_t)n_replicas));
   CCHK(dalloc(ctx, ctx->tab_d, 1));
   CCHK(dalloc(ctx, ctx->ctl_d, 1));
-  CCHK(dalloc(ctx, ctx->dispmax_d, (size_t)n_replicas));
   CCHK(dalloc(ctx, ctx->nlupd_d, (size_t)n_replicas));
   CCHK(dalloc(ctx, ctx->red, (size_t)n_replicas));
   CCHK(dalloc(ctx, ctx->x[0], (size_t)(N * D)));
@@ -985,8 +1102,7 @@ DP_API int dp_create(dp_ctx** out, int device, int dim, int64_t n_total, int n_r
     std::memset(&c, 0, sizeof(c));
     ok = ok && hipMemcpy(ctx->ctl_d, &c, sizeof(c), hipMemcpyHostToDevice) == hipSuccess;
     std::vector<u64> z64((size_t)n_replicas, 0);
-    ok = ok && hipMemcpy(ctx->dispmax_d, z64.data(), 8 * n_replicas, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(ctx->nlupd_d, z64.data(), 8 * n_replicas, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(ctx->nlupd_d, z64.data(), 8 * n_replicas, hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) return bail(ctx->fail(DP_ERR_HIP, "initialising device buffers failed"));
   }
   ctx->offsets = {0, n_total};
@@ -1009,13 +1125,13 @@ DP_API int dp_destroy(dp_ctx* ctx) {
   for (auto& ev : ctx->ev_pending) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
   for (auto& ev : ctx->ev_free) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
   if (ctx->mod) (void)hipModuleUnload(ctx->mod);
-  dfree(ctx->reps_d); dfree(ctx->tab_d); dfree(ctx->ctl_d); dfree(ctx->dispmax_d); dfree(ctx->nlupd_d);
+  dfree(ctx->reps_d); dfree(ctx->tab_d); dfree(ctx->ctl_d); dfree(ctx->nlupd_d);
   dfree(ctx->red); dfree(ctx->x[0]); dfree(ctx->x[1]); dfree(ctx->xq[0]); dfree(ctx->xq[1]);
   dfree(ctx->v); dfree(ctx->beta); dfree(ctx->mass); dfree(ctx->xref); dfree(ctx->F); dfree(ctx->E);
   dfree(ctx->W); dfree(ctx->LAP); dfree(ctx->xi); dfree(ctx->img); dfree(ctx->type); dfree(ctx->orig);
   dfree(ctx->rep); dfree(ctx->cellpack); dfree(ctx->cell_of); dfree(ctx->rank); dfree(ctx->tmp);
   dfree(ctx->perm); dfree(ctx->nlcnt); dfree(ctx->ccount); dfree(ctx->cstart); dfree(ctx->tiles);
-  dfree(ctx->nl); dfree(ctx->scratch); dfree(ctx->stage);
+  dfree(ctx->nl); dfree(ctx->tiletab); dfree(ctx->scratch); dfree(ctx->stage);
   if (ctx->ctl_h) (void)hipHostFree(ctx->ctl_h);
   if (ctx->st) (void)hipStreamDestroy(ctx->st);
   delete ctx;
@@ -1364,13 +1480,13 @@ DP_API int dp_set_seed(dp_ctx* ctx, uint64_t seed, uint64_t step_counter) {
 namespace {
 template <int D>
 int launch_export_t(dp_ctx* ctx, i32* out) {
-  const int g = grid_for(ctx->N);
+  const int g = ctx->tile_grid;
   if (ctx->nlw == 16)
-    k_nl_export<D, u16><<<g, DP_BLOCK, 0, ctx->st>>>((const u16*)ctx->nl, ctx->nlcnt, ctx->orig, ctx->cell_of,
-                                                     ctx->cstart, ctx->rep, ctx->reps_d, ctx->N, ctx->K, ctx->ob, out);
+    k_nl_export<D, u16><<<g, DP_TILE_THREADS, 0, ctx->st>>>((const u16*)ctx->nl, ctx->nlcnt, ctx->orig, ctx->cstart,
+                                                            ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, out);
   else
-    k_nl_export<D, u32><<<g, DP_BLOCK, 0, ctx->st>>>((const u32*)ctx->nl, ctx->nlcnt, ctx->orig, ctx->cell_of,
-                                                     ctx->cstart, ctx->rep, ctx->reps_d, ctx->N, ctx->K, ctx->ob, out);
+    k_nl_export<D, u32><<<g, DP_TILE_THREADS, 0, ctx->st>>>((const u32*)ctx->nl, ctx->nlcnt, ctx->orig, ctx->cstart,
+                                                            ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, out);
   HIPCHK(hipGetLastError());
   return DP_OK;
 }
@@ -1417,6 +1533,7 @@ DP_API int dp_get_stats(dp_ctx* ctx, int64_t* nl_updates, int64_t* steps, int32_
   HIPCHK(hipStreamSynchronize(ctx->st));
   if (nl_updates) HIPCHK(hipMemcpy(nl_updates, ctx->nlupd_d, 8 * ctx->R, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(ctx->ctl_h, ctx->ctl_d, sizeof(DpCtl), hipMemcpyDeviceToHost));
+  reduce_ctl(ctx->ctl_h);
   if (steps) *steps = (int64_t)ctx->ctl_h->step_counter;
   if (max_count) *max_count = ctx->ctl_h->max_count;
   return DP_OK;

@@ -11,6 +11,8 @@
 //   #define DP_NLW <16|32>      neighbour-entry width in bits
 //   #define DP_OB <n>           offset bits of an entry (stencil index above)
 //   #define DP_PAIRPREC <32|64> mixed mode: precision of the in-range pair terms
+//   #define DP_CMASK <mask>     dimensions with a single cell (minimum image in
+//                               the pair loop); 0 in the usual case
 // followed by the SymPy-generated device functions
 //   template<class T> T dp_pot_u(T x);   u(x)
 //   template<class T> T dp_pot_f(T x);   f(x) = -u'(x)
@@ -35,11 +37,15 @@
 #ifndef DP_D
 #error "DP_D must be defined"
 #endif
+#ifndef DP_CMASK
+#define DP_CMASK 0
+#endif
 
 #define DP_N3 (DpPow3<DP_D>::value)
 #define DP_OMASK ((1u << DP_OB) - 1u)
 #define DP_PACK_BITS ((DP_D == 1) ? 31 : 32 / DP_D)
 #define DP_GROUPS (DP_TILE_THREADS / DP_TPP)
+#define DP_JMAX ((DP_D + DP_TPP - 1) / DP_TPP)  // dimensions per lane in the integrator
 
 #if DP_NLW == 16
 typedef u16 dp_ent_t;
@@ -49,11 +55,80 @@ typedef u32 dp_ent_t;
 
 __device__ __forceinline__ int dp_tt(int ti, int tj) { return ti * DP_NTYPES + tj; }
 
+// Per type-pair constants of the inner loops, hoisted out of them: held in
+// registers for one type, in an LDS table for several.
+struct DpPC {
+  double cut;   // s > cut  <=>  x > r_cut (FP64 s)
+  double sig, eps, eos;  // sigma, eps, eps / sigma
+  double ks, ke;         // mixed FP64 terms: 2^-2q / sigma^2, eps / sigma * 2^-q
+  float ghi, glo;        // mixed screening guards in fixed-point units^2
+  float ksf, kef;        // FP32 terms: 2^-2q / sigma^2, eps / sigma * 2^-q
+  float epsf, sigf;
+};
+
+__device__ __forceinline__ DpPC dp_pc(const DpPairTab& T, int tt) {
+  DpPC c;
+  c.cut = T.cut_thresh[tt];
+  c.sig = T.sigma[tt];
+  c.eps = T.eps[tt];
+  c.eos = T.eps_over_sigma[tt];
+  c.ks = T.qscale2 * T.inv_sigma2[tt];
+  c.ke = T.eps_over_sigma[tt] * T.qscale;
+  c.ghi = T.guard_hi_q[tt];
+  c.glo = T.guard_lo_q[tt];
+  c.ksf = T.inv_sigma2_qf[tt];
+  c.kef = T.eps_over_sigma_qf[tt];
+  c.epsf = T.eps_f[tt];
+  c.sigf = T.sigma_f[tt];
+  return c;
+}
+
+#if DP_NTYPES == 1
+#define DP_PC_DECL const DpPC pc0 = dp_pc(*a.tab, 0);
+#define DP_PC(tt) pc0
+#else
+#define DP_PC_DECL
+#define DP_PC(tt) spc[tt]
+#endif
+
+
 template <class T>
 __device__ __forceinline__ T dp_group_sum(T v) {
 #pragma unroll
   for (int off = DP_TPP / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off);
   return v;
+}
+
+// Neighbour rows: stride K rounded up to 8 entries, so that every lane reads
+// its entries 8 bytes at a time (4 16-bit or 2 32-bit entries).
+__device__ __forceinline__ i64 dp_row_stride(int K) { return (i64)((K + 7) & ~7); }
+#define DP_EPL (8 / (int)sizeof(dp_ent_t))  // entries per 8-byte load
+
+// Call fn(idx, nvalid) for the 8-byte chunks of a row of length cnt that
+// belong to lane `sub` of its group: entries [EPL * (sub + j * TPP), + EPL)
+// for j = 0, 1, ...; idx[c] for c >= nvalid is unspecified.  The next chunk is
+// loaded while the current one is processed.
+template <class Fn>
+__device__ __forceinline__ void dp_for_chunks(const dp_ent_t* row, int cnt, int sub, Fn& fn) {
+  const int step = DP_EPL * DP_TPP;
+  int k0 = sub * DP_EPL;
+  uint2 nxt = make_uint2(0u, 0u);
+  if (k0 < cnt) nxt = *(const uint2*)(row + k0);
+  for (; k0 < cnt; k0 += step) {
+    const uint2 cur = nxt;
+    if (k0 + step < cnt) nxt = *(const uint2*)(row + k0 + step);
+    int idx[DP_EPL];
+#if DP_NLW == 16
+    idx[0] = (int)(cur.x & 0xffffu);
+    idx[1] = (int)(cur.x >> 16);
+    idx[2] = (int)(cur.y & 0xffffu);
+    idx[3] = (int)(cur.y >> 16);
+#else
+    idx[0] = (int)cur.x;
+    idx[1] = (int)cur.y;
+#endif
+    fn(idx, min(DP_EPL, cnt - k0));
+  }
 }
 
 // XCD-aware tile order: workgroup b runs on XCD b % 8; the tiles of XCD k are
@@ -68,128 +143,146 @@ __device__ __forceinline__ i64 dp_tile_of(i64 j, i64 ntiles) {
   return xcd * chunk + lane_in_xcd + j * per_xcd;
 }
 
+// One mixed-precision record: DP_RS ints (fixed-point coordinates, type,
+// padding), loaded and stored as one vector.
+struct DpRec {
+  i32 v[DP_RS];
+};
+__device__ __forceinline__ DpRec dp_ld_rec(const i32* p) {
+  DpRec r;
+#if DP_RS == 2
+  const int2 t = *(const int2*)p;
+  r.v[0] = t.x; r.v[1] = t.y;
+#elif DP_RS == 4
+  const int4 t = *(const int4*)p;
+  r.v[0] = t.x; r.v[1] = t.y; r.v[2] = t.z; r.v[3] = t.w;
+#else
+  const int4 t0 = *(const int4*)p;
+  const int4 t1 = *(const int4*)(p + 4);
+  r.v[0] = t0.x; r.v[1] = t0.y; r.v[2] = t0.z; r.v[3] = t0.w;
+  r.v[4] = t1.x; r.v[5] = t1.y; r.v[6] = t1.z; r.v[7] = t1.w;
+#endif
+  return r;
+}
+__device__ __forceinline__ void dp_st_rec(i32* p, const DpRec& r) {
+#if DP_RS == 2
+  *(int2*)p = make_int2(r.v[0], r.v[1]);
+#elif DP_RS == 4
+  *(int4*)p = make_int4(r.v[0], r.v[1], r.v[2], r.v[3]);
+#else
+  *(int4*)p = make_int4(r.v[0], r.v[1], r.v[2], r.v[3]);
+  *(int4*)(p + 4) = make_int4(r.v[4], r.v[5], r.v[6], r.v[7]);
+#endif
+}
+
+// Record of the particle in halo cell u at slot `slot`, with its coordinates
+// moved from its cell's centre to the tile's first home cell centre.
+__device__ __forceinline__ DpRec dp_halo_rec(const DpPairArgs& a, const DpTileSmem<DP_D>& S, const DpRep& P,
+                                             int u, i64 slot) {
+  DpRec r = dp_ld_rec(a.xq + slot * DP_RS);
+#pragma unroll
+  for (int d = 0; d < DP_D; ++d) r.v[d] = (i32)((u32)r.v[d] + dp_halo_shift<DP_D>(S, P, u, d));
+  return r;
+}
+
+// Slot of staged halo record idx.
+__device__ __forceinline__ i64 dp_slot_of(const DpTileSmem<DP_D>& S, int nh, int idx) {
+  const int u = dp_halo_of<DP_D>(S, nh, idx);
+  return (i64)S.hstart[u] + (idx - S.hbase[u]);
+}
+
+// Per-particle pass over the home particles of the staged tile.  LDS selects
+// where neighbour data come from: the staged halo (LDS = true) or global
+// memory through the tile tables (tiles whose halo exceeds the LDS capacity).
 // KIND: 0 forces, 1 observables (forces + per-particle energy / virial /
 // Laplacian), 2 fused force + G-JF, 3 fused force + leap-frog.
-template <int KIND>
-__device__ void dp_tile_run(const DpPairArgs& a) {
-  __shared__ DpTileSmem<DP_D> S;
-#if DP_MODE == 1
-  __shared__ i32 shq[DP_N3 * DP_D];
-  extern __shared__ i32 srec[];  // cap * DP_RS
-#else
-  extern __shared__ double sxd[];  // cap * DP_D (+ cap ints of types)
-#endif
+template <int KIND, bool LDS>
+__device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpTileSmem<DP_D>& S, const DpRep& P,
+                                                  const void* stage, const DpPC* spc, bool& trip) {
   const DpPairTab& T = *a.tab;
+  (void)spc;
+  DP_PC_DECL
   const int tid = threadIdx.x;
   const int group = tid / DP_TPP;
   const int sub = tid % DP_TPP;
   const dp_ent_t* nl = (const dp_ent_t*)a.nl;
-  double dmax = 0.0;  // running maximum displacement^2 of this lane's particles
-  int cur_rep = -1;
-  const i64 chunk = (a.ntiles + 7) / 8;
-  const i64 xcd = blockIdx.x & 7;
-  const i64 cend = (xcd + 1) * chunk < a.ntiles ? (xcd + 1) * chunk : a.ntiles;
-  for (i64 j = 0;; ++j) {
-    const i64 t = dp_tile_of(j, a.ntiles);
-    if (t >= cend) break;
-    dp_tile_setup<DP_D>(S, a.reps, a.R, a.cstart, t, a.cap);
-    const int r = S.rep;
-    const DpRep& P = a.reps[r];
-    if ((KIND == 2 || KIND == 3) && r != cur_rep) {
-      // flush the displacement maximum of the previous replica
-      if (cur_rep >= 0) {
-        double m = dmax;
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-        if ((tid & 63) == 0) atomicMax(&a.dispmax[cur_rep], (unsigned long long)__double_as_longlong(m));
-      }
-      dmax = 0.0;
-      cur_rep = r;
-    }
-    const int nh = P.nhalo;
-    const bool fits = S.fits != 0;
+  const int nh = P.nhalo;
 #if DP_MODE == 1
-    for (int q = tid; q < DP_N3 * DP_D; q += blockDim.x) {
-      const int st = q / DP_D, d = q % DP_D;
-      int sd[DP_D];
-      dp_stencil_digits<DP_D>(st, P.order, sd);
-      shq[q] = (P.nc[d] == 1) ? 0 : -sd[d] * P.Wq[d];
-    }
-    if (fits) {
-      for (int idx = tid; idx < S.nrec; idx += blockDim.x) {
-        const int u = dp_halo_of<DP_D>(S, nh, idx);
-        const i64 slot = S.hstart[u] + (idx - S.hbase[u]);
-#pragma unroll
-        for (int q = 0; q < DP_RS; ++q) srec[idx * DP_RS + q] = a.xq[slot * DP_RS + q];
-      }
-    }
+  const i32* srec = (const i32*)stage;
 #else
-    i32* stype = (i32*)(sxd + (size_t)a.cap * DP_D);
-    if (fits) {
-      for (int idx = tid; idx < S.nrec; idx += blockDim.x) {
-        const int u = dp_halo_of<DP_D>(S, nh, idx);
-        const i64 slot = S.hstart[u] + (idx - S.hbase[u]);
-#pragma unroll
-        for (int d = 0; d < DP_D; ++d) sxd[idx * DP_D + d] = a.x[slot * DP_D + d];
-        if (DP_NTYPES > 1) stype[idx] = a.type[slot];
-      }
-    }
+  const double* sxd = (const double*)stage;
+  const i32* stype = (const i32*)(sxd + (size_t)a.cap * DP_D);
 #endif
-    __syncthreads();
-    for (int base = 0; base < S.nhomep; base += DP_GROUPS) {
-      const int k = base + group;
-      const bool active = k < S.nhomep;
-      int uh = 0, loc = 0;
-      if (active) {
-        const int m = dp_home_of<DP_D>(S, P.nhome, k);
-        uh = S.home_u[m];
-        loc = k - S.home_p[m];
-      }
-      const i64 slot_i = active ? (i64)S.hstart[uh] + loc : 0;
-      const int self = S.hbase[uh] + loc;
-      // rows longer than K were truncated at the emit (and reported as overflow)
-      const int cnt = active ? min(a.nlcnt[slot_i], a.K) : 0;
-      const dp_ent_t* row = nl + slot_i * a.K;
-      double f[DP_D];
+  for (int base = 0; base < S.nhomep; base += DP_GROUPS) {
+    const int k = base + group;
+    const bool active = k < S.nhomep;
+    int uh = 0, loc = 0;
+    if (active) {
+      const int m = dp_home_of<DP_D>(S, P.nhome, k);
+      uh = S.home_u[m];
+      loc = k - S.home_p[m];
+    }
+    const i64 slot_i = active ? (i64)S.hstart[uh] + loc : 0;
+    const int self = S.hbase[uh] + loc;
+    // rows longer than K were truncated at the emit (and reported as overflow)
+    const int cnt = active ? min(a.nlcnt[slot_i], a.K) : 0;
+    const dp_ent_t* row = nl + slot_i * dp_row_stride(a.K);
+    double f[DP_D];
 #pragma unroll
-      for (int d = 0; d < DP_D; ++d) f[d] = 0.0;
-      double e = 0.0, w = 0.0, lap = 0.0;
+    for (int d = 0; d < DP_D; ++d) f[d] = 0.0;
+    double e = 0.0, w = 0.0, lap = 0.0;
 #if DP_MODE == 1
-      // ---------------- mixed: fixed-point displacements ----------------
-      i32 qi[DP_D];
-      int ti = 0;
+    // ---------------- mixed: fixed-point displacements ----------------
+    const DpRec ri = LDS ? dp_ld_rec(srec + self * DP_RS) : dp_halo_rec(a, S, P, uh, slot_i);
+    const int ti = (DP_NTYPES > 1) ? ri.v[DP_D] : 0;
+    // one chunk of DP_EPL entries at a time, branch-free apart from the rare
+    // exact decision near the cut: records first, then independent pair terms
+    auto chunk = [&](const int (&idx)[DP_EPL], const int nvalid) {
+      DpRec rj[DP_EPL];
 #pragma unroll
-      for (int d = 0; d < DP_D; ++d) qi[d] = fits ? srec[self * DP_RS + d] : a.xq[slot_i * DP_RS + d];
-      if (DP_NTYPES > 1) ti = fits ? srec[self * DP_RS + DP_D] : a.xq[slot_i * DP_RS + DP_D];
-      for (int kk = sub; kk < cnt; kk += DP_TPP) {
-        const u32 ent = (u32)row[kk];
-        const int st = (int)(ent >> DP_OB);
-        const int off = (int)(ent & DP_OMASK);
-        const int u = uh + S.st_off[st];
-        const int idx = S.hbase[u] + off;
-        i32 dq[DP_D];
-        float s2 = 0.f;
-        int tj = 0;
+      for (int c = 0; c < DP_EPL; ++c) {
+        const int ic = (c < nvalid) ? idx[c] : self;  // padding entries read the particle itself
+        if (LDS) {
+          rj[c] = dp_ld_rec(srec + ic * DP_RS);
+        } else {
+          const int u = dp_halo_of<DP_D>(S, nh, ic);
+          rj[c] = dp_halo_rec(a, S, P, u, (i64)S.hstart[u] + (ic - S.hbase[u]));
+        }
+      }
+      i32 dq[DP_EPL][DP_D];
+      float dqf[DP_EPL][DP_D];
+      float s2[DP_EPL];
+      int tt[DP_EPL];
+      bool in[DP_EPL];
+      bool near = false;
+#pragma unroll
+      for (int c = 0; c < DP_EPL; ++c) {
+        s2[c] = 0.f;
 #pragma unroll
         for (int d = 0; d < DP_D; ++d) {
-          const i32 qj = fits ? srec[idx * DP_RS + d] : a.xq[((i64)S.hstart[u] + off) * DP_RS + d];
-          i32 dd = (i32)((u32)qi[d] - (u32)qj + (u32)shq[st * DP_D + d]);
-          if (P.nc[d] == 1) {
+          i32 dd = (i32)((u32)ri.v[d] - (u32)rj[c].v[d]);
+          if (DP_CMASK & (1 << d)) {
+            // single cell along d (offsets from the box centre): minimum image
+            // (a no-op for other replicas, whose displacements are < L/2)
             const i32 h = P.Lq[d] >> 1;
-            if (dd > h) dd -= P.Lq[d];
-            else if (dd < -h) dd += P.Lq[d];
+            dd = (dd > h) ? dd - P.Lq[d] : ((dd < -h) ? dd + P.Lq[d] : dd);
           }
-          dq[d] = dd;
-          const float df = (float)dd;
-          s2 = fmaf(df, df, s2);
+          dq[c][d] = dd;
+          dqf[c][d] = (float)dd;
+          s2[c] = fmaf(dqf[c][d], dqf[c][d], s2[c]);
         }
-        if (DP_NTYPES > 1) tj = fits ? srec[idx * DP_RS + DP_D] : a.xq[((i64)S.hstart[u] + off) * DP_RS + DP_D];
-        const int tt = dp_tt(ti, tj);
-        if (s2 >= T.guard_hi_q[tt]) continue;  // certainly beyond the cut
-        if (s2 > T.guard_lo_q[tt]) {
-          // too close to the cut to decide from the fixed-point values:
-          // decide with the reference's FP64 operations (rare)
-          const i64 slot_j = (i64)S.hstart[u] + off;
+        tt[c] = dp_tt(ti, (DP_NTYPES > 1) ? rj[c].v[DP_D] : 0);
+        const DpPC& pc = DP_PC(tt[c]);
+        in[c] = (c < nvalid) && (s2[c] < pc.ghi);  // else certainly beyond the cut
+        near |= in[c] && (s2[c] > pc.glo);
+      }
+      if (near) {
+        // too close to the cut to decide from the fixed-point values: decide
+        // with the reference's FP64 operations (rare)
+#pragma unroll
+        for (int c = 0; c < DP_EPL; ++c) {
+          if (!(in[c] && s2[c] > DP_PC(tt[c]).glo)) continue;
+          const i64 slot_j = dp_slot_of(S, nh, idx[c]);
           double s = 0.0;
 #pragma unroll
           for (int d = 0; d < DP_D; ++d) {
@@ -197,191 +290,280 @@ __device__ void dp_tile_run(const DpPairArgs& a) {
                                              P.halfL[d]);
             s = (d == 0) ? dp_mul(tq, tq) : dp_add(s, dp_mul(tq, tq));
           }
-          if (s > T.cut_thresh[tt]) continue;
+          in[c] = !(s > DP_PC(tt[c]).cut);
         }
+      }
+#if DP_PAIRPREC == 32
+      float fc[DP_D];  // FP32 partial sums over the chunk, added to the FP64 totals below
+#pragma unroll
+      for (int d = 0; d < DP_D; ++d) fc[d] = 0.f;
+#endif
+#pragma unroll
+      for (int c = 0; c < DP_EPL; ++c) {
 #if DP_PAIRPREC == 64
+        const DpPC& pc = DP_PC(tt[c]);
         double s64 = 0.0;
 #pragma unroll
-        for (int d = 0; d < DP_D; ++d) s64 = fma((double)dq[d], (double)dq[d], s64);
-        const double sx = s64 * T.qscale2 * T.inv_sigma2[tt];
+        for (int d = 0; d < DP_D; ++d) s64 = fma((double)dq[c][d], (double)dq[c][d], s64);
+        // entries off the list (padding, beyond the cut) evaluate at s = 1
+        const double sx = in[c] ? s64 * pc.ks : 1.0;
 #if DP_EVEN
         const double g = dp_pot_gs<double>(sx);  // f(x)/x
 #else
         const double xr = sqrt(sx);
         const double g = dp_pot_f<double>(xr) * dp_rcp(xr);
 #endif
-        const double fs = T.eps_over_sigma[tt] * g * T.qscale;  // eps f(x)/r per fixed-point unit
+        const double fs = in[c] ? pc.ke * g : 0.0;  // eps f(x)/r per fixed-point unit
 #pragma unroll
-        for (int d = 0; d < DP_D; ++d) f[d] = fma(fs, (double)dq[d], f[d]);
-        if (KIND == 1) {
-          const double eps = T.eps[tt];
+        for (int d = 0; d < DP_D; ++d) f[d] = fma(fs, (double)dq[c][d], f[d]);
+        if (KIND == 1 && in[c]) {
 #if DP_EVEN
-          e += eps * (dp_pot_us<double>(sx) - T.u_rc);
+          e += pc.eps * (dp_pot_us<double>(sx) - T.u_rc);
 #else
-          e += eps * (dp_pot_u<double>(xr) - T.u_rc);
+          e += pc.eps * (dp_pot_u<double>(xr) - T.u_rc);
 #endif
-          w += eps * T.sigma[tt] * g * sx;
-          lap += eps * dp_pot_c<double>(sqrt(sx)) - (double)(DP_D - 1) * T.eps_over_sigma[tt] * g;
+          w += pc.eps * pc.sig * g * sx;
+          lap += pc.eps * dp_pot_c<double>(sqrt(sx)) - (double)(DP_D - 1) * pc.eos * g;
         }
 #else
-        const float sxf = s2 * T.inv_sigma2_qf[tt];
+        const DpPC& pc = DP_PC(tt[c]);
+        const float sxf = in[c] ? s2[c] * pc.ksf : 1.0f;
 #if DP_EVEN
         const float g = dp_pot_gs<float>(sxf);
 #else
         const float xrf = sqrtf(sxf);
         const float g = dp_pot_f<float>(xrf) * dp_rcp(xrf);
 #endif
-        const float fs = T.eps_over_sigma_qf[tt] * g;
+        const float fs = in[c] ? pc.kef * g : 0.0f;
 #pragma unroll
-        for (int d = 0; d < DP_D; ++d) f[d] += (double)(fs * (float)dq[d]);
-        if (KIND == 1) {
-          const float eps = T.eps_f[tt];
+        for (int d = 0; d < DP_D; ++d) fc[d] = fmaf(fs, dqf[c][d], fc[d]);
+        if (KIND == 1 && in[c]) {
 #if DP_EVEN
-          e += (double)(eps * dp_pot_us<float>(sxf)) - (double)eps * T.u_rc;
+          e += (double)(pc.epsf * dp_pot_us<float>(sxf)) - (double)pc.epsf * T.u_rc;
 #else
-          e += (double)(eps * dp_pot_u<float>(xrf)) - (double)eps * T.u_rc;
+          e += (double)(pc.epsf * dp_pot_u<float>(xrf)) - (double)pc.epsf * T.u_rc;
 #endif
-          w += (double)(eps * T.sigma_f[tt] * g * sxf);
-          lap += (double)(eps * dp_pot_c<float>(sqrtf(sxf))) -
-                 (double)(DP_D - 1) * (double)(T.eps_f[tt] / T.sigma_f[tt] * g);
+          w += (double)(pc.epsf * pc.sigf * g * sxf);
+          lap += (double)(pc.epsf * dp_pot_c<float>(sqrtf(sxf))) -
+                 (double)(DP_D - 1) * (double)(pc.epsf / pc.sigf * g);
         }
 #endif
       }
-#else
-      // ---------------- FP64 parity mode ----------------
-      double xi_[DP_D];
+#if DP_PAIRPREC == 32
 #pragma unroll
-      for (int d = 0; d < DP_D; ++d) xi_[d] = fits ? sxd[self * DP_D + d] : a.x[slot_i * DP_D + d];
-      const int ti = (DP_NTYPES > 1) ? (fits ? stype[self] : a.type[slot_i]) : 0;
-      for (int kk = sub; kk < cnt; kk += DP_TPP) {
-        const u32 ent = (u32)row[kk];
-        const int st = (int)(ent >> DP_OB);
-        const int off = (int)(ent & DP_OMASK);
-        const int u = uh + S.st_off[st];
-        const int idx = S.hbase[u] + off;
-        const i64 slot_j = (i64)S.hstart[u] + off;
+      for (int d = 0; d < DP_D; ++d) f[d] += (double)fc[d];
+#endif
+    };
+    dp_for_chunks(row, cnt, sub, chunk);
+#else
+    // ---------------- FP64 parity mode ----------------
+    double xi_[DP_D];
+#pragma unroll
+    for (int d = 0; d < DP_D; ++d) xi_[d] = LDS ? sxd[self * DP_D + d] : a.x[slot_i * DP_D + d];
+    const int ti = (DP_NTYPES > 1) ? (LDS ? stype[self] : a.type[slot_i]) : 0;
+    // one chunk of DP_EPL entries at a time: positions first, then the
+    // reference's FP64 operations, accumulated in row order within the lane
+    auto chunk = [&](const int (&idx)[DP_EPL], const int nvalid) {
+      double xj[DP_EPL][DP_D];
+      int tj[DP_EPL];
+#pragma unroll
+      for (int c = 0; c < DP_EPL; ++c) {
+        const int ic = (c < nvalid) ? idx[c] : self;
+        tj[c] = 0;
+        if (LDS) {
+#pragma unroll
+          for (int d = 0; d < DP_D; ++d) xj[c][d] = sxd[ic * DP_D + d];
+          if (DP_NTYPES > 1) tj[c] = stype[ic];
+        } else {
+          const i64 slot_j = dp_slot_of(S, nh, ic);
+#pragma unroll
+          for (int d = 0; d < DP_D; ++d) xj[c][d] = a.x[slot_j * DP_D + d];
+          if (DP_NTYPES > 1) tj[c] = a.type[slot_j];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < DP_EPL; ++c) {
         double dd[DP_D];
         double s = 0.0;
 #pragma unroll
         for (int d = 0; d < DP_D; ++d) {
-          const double xj = fits ? sxd[idx * DP_D + d] : a.x[slot_j * DP_D + d];
-          const double tq = dp_minimage_gt(dp_sub(xi_[d], xj), P.L[d], P.halfL[d]);
+          const double tq = dp_minimage_gt(dp_sub(xi_[d], xj[c][d]), P.L[d], P.halfL[d]);
           dd[d] = tq;
           s = (d == 0) ? dp_mul(tq, tq) : dp_add(s, dp_mul(tq, tq));
         }
-        const int tj = (DP_NTYPES > 1) ? (fits ? stype[idx] : a.type[slot_j]) : 0;
-        const int tt = dp_tt(ti, tj);
-        if (s > T.cut_thresh[tt]) continue;  // x > r_cut  <=>  s > threshold
-        const double rr = sqrt(s);
-        const double xr = rr / T.sigma[tt];
-        const double eps = T.eps[tt];
+        const DpPC& pc = DP_PC(dp_tt(ti, tj[c]));
+        // x > r_cut  <=>  s > threshold; off-list entries evaluate at s = 1
+        const bool in = (c < nvalid) && !(s > pc.cut);
+        const double rr = sqrt(in ? s : 1.0);
+        const double xr = rr / pc.sig;
+        const double eps = pc.eps;
         const double sf = eps * dp_pot_f<double>(xr);
 #pragma unroll
-        for (int d = 0; d < DP_D; ++d) f[d] = f[d] + sf * (dd[d] / rr);
-        if (KIND == 1) {
+        for (int d = 0; d < DP_D; ++d) f[d] = in ? f[d] + sf * (dd[d] / rr) : f[d];
+        if (KIND == 1 && in) {
           e += eps * (dp_pot_u<double>(xr) - T.u_rc);
           w += sf * rr;
           lap += eps * dp_pot_c<double>(xr) + (double)(DP_D - 1) * (-sf) / rr;
         }
       }
+    };
+    dp_for_chunks(row, cnt, sub, chunk);
 #endif
-      // combine the partial sums of the particle's DP_TPP lanes
+    // combine the partial sums of the particle's DP_TPP lanes (every lane
+    // of the group ends with the totals)
 #pragma unroll
-      for (int d = 0; d < DP_D; ++d) f[d] = dp_group_sum(f[d]);
-      if (KIND == 1) {
-        e = dp_group_sum(e);
-        w = dp_group_sum(w);
-        lap = dp_group_sum(lap);
-      }
-      if (!active || sub != 0) continue;
-      if (KIND == 0 || KIND == 1) {
-        if (a.F) {
+    for (int d = 0; d < DP_D; ++d) f[d] = dp_group_sum(f[d]);
+    if (KIND == 1) {
+      e = dp_group_sum(e);
+      w = dp_group_sum(w);
+      lap = dp_group_sum(lap);
+    }
+    if (!active) continue;
+    if (KIND == 0 || KIND == 1) {
+      if (a.F) {
 #pragma unroll
-          for (int d = 0; d < DP_D; ++d) a.F[slot_i * DP_D + d] = f[d];
-        }
-        if (KIND == 1) {
-          if (a.E) a.E[slot_i] = e;
-          if (a.W) a.W[slot_i] = w;
-          if (a.LAP) a.LAP[slot_i] = lap;
-        }
-        continue;
+        for (int d = 0; d < DP_D; ++d)
+          if (d % DP_TPP == sub) a.F[slot_i * DP_D + d] = f[d];
       }
-      // fused integrator update of particle slot_i (one lane per particle)
-      {
+      if (KIND == 1 && sub == 0) {
+        if (a.E) a.E[slot_i] = e;
+        if (a.W) a.W[slot_i] = w;
+        if (a.LAP) a.LAP[slot_i] = lap;
+      }
+      continue;
+    }
+    // fused integrator update of particle slot_i: lane `sub` of the group
+    // updates dimensions sub, sub + DP_TPP, ...; the normal numbers are drawn
+    // in one SIMD pass when DP_TPP >= DP_D
+    {
 #pragma clang fp contract(off)
-        const i64 ii = slot_i;
-        const double m = a.mass[ii];
-        const double dt = P.dt;
-        const u64 step = a.ctl->step_index;
+      const i64 ii = slot_i;
+      const double m = a.mass[ii];
+      const double dt = P.dt;
+      double z_own[DP_JMAX];
+      if (KIND == 2) {
         const u32 pid = (u32)a.orig[ii];
-        double xn[DP_D];
-        if (KIND == 2) {
-          // integrator.py:17-40 (G-JF), numpy's left-to-right evaluation order
-          const double alpha = m / P.tau;
-          const double bvar = 2.0 * alpha * P.T_ * dt;
-          const double sq = sqrt(bvar);
-          const double qq = alpha * dt / 2.0 / m;
-          const double num = 1.0 - qq;
-          const double den = 1.0 + qq;
-          const double ca = num / den;
-          const double cb = 1.0 / den;
 #pragma unroll
-          for (int d = 0; d < DP_D; ++d) {
-            const double z = a.xi ? a.xi[(i64)pid * DP_D + d] : dp_normal(a.seed, step, pid, d);
-            const double nb = sq * z;
-            const double v0 = a.v[ii * DP_D + d];
-            const double b0 = a.beta[ii * DP_D + d];
-            const double v1 = ca * v0 + cb * dt * f[d] / m + cb * (b0 + nb) / 2.0 / m;
-            xn[d] = a.x[ii * DP_D + d] + v1 * dt;
-            a.v[ii * DP_D + d] = v1;
-            a.beta[ii * DP_D + d] = nb;
-          }
+        for (int j = 0; j < DP_JMAX; ++j) {
+          const int d = sub + j * DP_TPP;
+          z_own[j] = 0.0;
+          if (d < DP_D) z_own[j] = a.xi ? a.xi[(i64)pid * DP_D + d] : dp_normal(a.seed, a.ctl->step_index, pid, d);
+        }
+      }
+      // integrator.py:17-40 (G-JF) coefficients, numpy's left-to-right order
+      const double alpha = m / P.tau;
+      const double bvar = 2.0 * alpha * P.T_ * dt;
+      const double sq = sqrt(bvar);
+      const double qq = alpha * dt / 2.0 / m;
+      const double num = 1.0 - qq;
+      const double den = 1.0 + qq;
+      const double ca = num / den;
+      const double cb = 1.0 / den;
+#if DP_MODE == 1
+      const u32 pack = (u32)a.cellpack[ii];
+#endif
+      double tq_own[DP_JMAX];
+#pragma unroll
+      for (int j = 0; j < DP_JMAX; ++j) tq_own[j] = 0.0;
+#pragma unroll
+      for (int d = 0; d < DP_D; ++d) {
+        if (d % DP_TPP != sub) continue;
+        double xn;
+        if (KIND == 2) {
+          const double nb = sq * z_own[d / DP_TPP];
+          const double v0 = a.v[ii * DP_D + d];
+          const double b0 = a.beta[ii * DP_D + d];
+          const double v1 = ca * v0 + cb * dt * f[d] / m + cb * (b0 + nb) / 2.0 / m;
+          xn = a.x[ii * DP_D + d] + v1 * dt;
+          a.v[ii * DP_D + d] = v1;
+          a.beta[ii * DP_D + d] = nb;
         } else {
           // integrator.py:5-14 (leap-frog)
-#pragma unroll
-          for (int d = 0; d < DP_D; ++d) {
-            const double v1 = a.v[ii * DP_D + d] + dt * f[d] / m;
-            xn[d] = a.x[ii * DP_D + d] + dt * v1;
-            a.v[ii * DP_D + d] = v1;
-          }
+          const double v1 = a.v[ii * DP_D + d] + dt * f[d] / m;
+          xn = a.x[ii * DP_D + d] + dt * v1;
+          a.v[ii * DP_D + d] = v1;
         }
         // positions.py:90-98 wrap, then neighbor_list.py:183-201 displacement
-        double sdisp = 0.0;
-#pragma unroll
-        for (int d = 0; d < DP_D; ++d) {
-          if (a.wrap) {
-            const double sh = floor(xn[d] / P.L[d]);
-            a.img[ii * DP_D + d] += (i32)sh;
-            xn[d] = xn[d] - sh * P.L[d];
-          }
-          a.x_out[ii * DP_D + d] = xn[d];
-          double tq = dp_sub(xn[d], a.xref[ii * DP_D + d]);
-          tq = dp_minimage_gt(tq, P.L[d], P.halfL[d]);
-          sdisp = (d == 0) ? dp_mul(tq, tq) : dp_add(sdisp, dp_mul(tq, tq));
+        if (a.wrap) {
+          const double sh = floor(xn / P.L[d]);
+          a.img[ii * DP_D + d] += (i32)sh;
+          xn = xn - sh * P.L[d];
         }
-        if (sdisp != sdisp) sdisp = __longlong_as_double(0x7ff0000000000000ll);
-        dmax = fmax(dmax, sdisp);
+        a.x_out[ii * DP_D + d] = xn;
+        tq_own[d / DP_TPP] = dp_minimage_gt(dp_sub(xn, a.xref[ii * DP_D + d]), P.L[d], P.halfL[d]);
 #if DP_MODE == 1
-        const u32 pack = (u32)a.cellpack[ii];
-#pragma unroll
-        for (int d = 0; d < DP_D; ++d) {
-          const u32 c = (pack >> (DP_PACK_BITS * d)) & ((1u << DP_PACK_BITS) - 1u);
-          const double cen = ((double)c + 0.5) * P.w[d];
-          a.xq_out[ii * DP_RS + d] = (i32)rint(dp_minimage_gt(xn[d] - cen, P.L[d], P.halfL[d]) * T.qinv);
-        }
-#pragma unroll
-        for (int q = DP_D; q < DP_RS; ++q) a.xq_out[ii * DP_RS + q] = a.xq[ii * DP_RS + q];
+        const u32 c = (pack >> (DP_PACK_BITS * d)) & ((1u << DP_PACK_BITS) - 1u);
+        const double cen = ((double)c + 0.5) * P.w[d];
+        a.xq_out[ii * DP_RS + d] = (i32)rint(dp_minimage_gt(xn - cen, P.L[d], P.halfL[d]) * T.qinv);
 #endif
       }
+#if DP_MODE == 1
+      if (sub == 0) {
+#pragma unroll
+        for (int q = DP_D; q < DP_RS; ++q) a.xq_out[ii * DP_RS + q] = a.xq[ii * DP_RS + q];
+      }
+#endif
+      // squared displacement summed in dimension order from the owners
+      const int gbase = (tid & 63) & ~(DP_TPP - 1);
+      double sdisp = 0.0;
+#pragma unroll
+      for (int d = 0; d < DP_D; ++d) {
+        const double tq = (DP_TPP == 1) ? tq_own[d] : __shfl(tq_own[d / DP_TPP], gbase + (d % DP_TPP));
+        sdisp = (d == 0) ? dp_mul(tq, tq) : dp_add(sdisp, dp_mul(tq, tq));
+      }
+      trip |= dp_skin_trips(sdisp, P.skin_half_sq);
+    }
+  }
+}
+
+template <int KIND>
+__device__ void dp_tile_run(const DpPairArgs& a) {
+  __shared__ DpTileSmem<DP_D> S;
+  extern __shared__ __align__(16) i32 stage_i[];  // mixed: cap * DP_RS ints; FP64: cap * DP_D doubles + cap types
+  const int tid = threadIdx.x;
+#if DP_NTYPES > 1
+  __shared__ DpPC spc[DP_NTYPES * DP_NTYPES];
+  if (tid < DP_NTYPES * DP_NTYPES) spc[tid] = dp_pc(*a.tab, tid);
+  // visible after the first tile's table-load barrier
+#else
+  const DpPC* spc = nullptr;
+#endif
+  bool trip = false;  // some particle of this lane moved beyond half the skin
+  const i64 chunk = (a.ntiles + 7) / 8;
+  const i64 xcd = blockIdx.x & 7;
+  const i64 cend = (xcd + 1) * chunk < a.ntiles ? (xcd + 1) * chunk : a.ntiles;
+  for (i64 j = 0;; ++j) {
+    const i64 t = dp_tile_of(j, a.ntiles);
+    if (t >= cend) break;
+    dp_tile_load<DP_D>(S, a.tiletab + t * (i64)a.tstride, a.cap);
+    const DpRep& P = a.reps[S.rep];
+    const int nh = P.nhalo;
+    const bool fits = S.fits != 0;
+    if (fits) {
+#if DP_MODE == 1
+      for (int idx = tid; idx < S.nrec; idx += blockDim.x) {
+        const int u = dp_halo_of<DP_D>(S, nh, idx);
+        dp_st_rec(stage_i + idx * DP_RS, dp_halo_rec(a, S, P, u, (i64)S.hstart[u] + (idx - S.hbase[u])));
+      }
+#else
+      double* sxd = (double*)stage_i;
+      i32* stype = (i32*)(sxd + (size_t)a.cap * DP_D);
+      for (int idx = tid; idx < S.nrec; idx += blockDim.x) {
+        const i64 slot = dp_slot_of(S, nh, idx);
+#pragma unroll
+        for (int d = 0; d < DP_D; ++d) sxd[idx * DP_D + d] = a.x[slot * DP_D + d];
+        if (DP_NTYPES > 1) stype[idx] = a.type[slot];
+      }
+#endif
     }
     __syncthreads();
+    if (fits)
+      dp_tile_particles<KIND, true>(a, S, P, stage_i, spc, trip);
+    else
+      dp_tile_particles<KIND, false>(a, S, P, stage_i, spc, trip);
+    __syncthreads();
   }
-  if ((KIND == 2 || KIND == 3) && cur_rep >= 0) {
-    double m = dmax;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-    if ((tid & 63) == 0) atomicMax(&a.dispmax[cur_rep], (unsigned long long)__double_as_longlong(m));
-  }
+  if (KIND == 2 || KIND == 3) dp_raise_rebuild(trip, a.ctl);
 }
 
 extern "C" __global__ void __launch_bounds__(DP_TILE_THREADS) dp_tile_force(DpPairArgs a) { dp_tile_run<0>(a); }
@@ -453,7 +635,6 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
         sdisp = (d == 0) ? dp_mul(tq, tq) : dp_add(sdisp, dp_mul(tq, tq));
       }
     }
-    if (sdisp != sdisp) sdisp = __longlong_as_double(0x7ff0000000000000ll);
 #if DP_MODE == 1
     if (a.wrap) {
       const u32 pack = (u32)a.cellpack[ii];
@@ -468,22 +649,7 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
     }
 #endif
   }
-  if (!a.wrap) return;
-  // skin-displacement maximum: one atomic per wavefront when the wavefront
-  // lies in one replica, per lane otherwise
-  const int lane = threadIdx.x & 63;
-  const unsigned long long b = __ballot(valid);
-  if (b == 0ull) return;
-  const int first = __ffsll((unsigned long long)b) - 1;
-  const int rf = __shfl(r, first);
-  if (__all((!valid) || r == rf)) {
-    double mx = valid ? sdisp : 0.0;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
-    if (lane == first) atomicMax(&a.dispmax[rf], (unsigned long long)__double_as_longlong(mx));
-  } else if (valid) {
-    atomicMax(&a.dispmax[r], (unsigned long long)__double_as_longlong(sdisp));
-  }
+  if (a.wrap) dp_raise_rebuild(valid && dp_skin_trips(sdisp, P.skin_half_sq), a.ctl);
 }
 
 extern "C" __global__ void __launch_bounds__(256) dp_integrate_gjf(DpPairArgs a) { dp_integrate_body<1>(a); }

@@ -170,6 +170,13 @@ int dp_get_kernel_time(dp_ctx* ctx, double* total_ms, int64_t* launches);
  * make_pair_potential's replacement (potential.py:43-66). */
 int dp_compile_check(int dim, int precision, int ntypes, int even, const char* device_src, int pair_precision,
                      char* log, int64_t log_len, int64_t* code_size);
+/* Same check for an explicit kernel shape: lanes per particle (1, 2, 4, 8,
+ * 16), neighbour-entry width (16 or 32 bits) and the mask of dimensions
+ * holding a single cell, the variants dp_set_tuning and the tile planner can
+ * select. */
+int dp_compile_check_tuned(int dim, int precision, int ntypes, int even, const char* device_src,
+                           int pair_precision, int threads_per_particle, int entry_bits, int collapsed_mask,
+                           char* log, int64_t log_len, int64_t* code_size);
 /* Same compile; copies the gfx950 code object into out when out_len is large
  * enough (code_size receives the size). */
 int dp_compile_code(int dim, int precision, int ntypes, int even, const char* device_src, int pair_precision,
