@@ -70,7 +70,7 @@ def workload(name, seed):
                 dt=0.005, T=T, tau=10.0, K=256, seed=seed)
 
 
-def make_engine(wl, precision, pair_precision="fp64"):
+def make_engine(wl, precision, pair_precision="fp64", tuning=None):
     from dompap_amd.engine import Engine
     from dompap_amd.potential import device_potential
     n, dim = wl["x"].shape
@@ -81,6 +81,8 @@ def make_engine(wl, precision, pair_precision="fp64"):
     eng.set_neighbor_params(1.0 * wl["r_cut"] + wl["skin"], wl["skin"], wl["K"])
     eng.set_integrator(wl["dt"], wl["T"], wl["tau"])
     eng.set_seed(wl["seed"], 0)
+    if tuning is not None:
+        eng.set_tuning(*tuning)
     eng.set_field("positions", wl["x"])
     eng.set_field("velocities", wl["v"])
     eng.set_field("masses", wl["m"])
@@ -188,6 +190,10 @@ def main():
     ap.add_argument("--pair-precision", default="fp64", choices=["fp64", "fp32"],
                     help="mixed mode: precision of the in-range pair terms")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--tpp", type=int, default=0, help="lanes per particle (0: planner's choice)")
+    ap.add_argument("--tile", type=int, default=0, help="home particles per tile (0: planner's choice)")
+    ap.add_argument("--fused", type=int, default=-1, choices=[-1, 0, 1],
+                    help="1: fused force + integrate kernel, 0: force kernel then integrate kernel, -1: planner")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-nve", action="store_true", help="skip the NVE drift segment")
@@ -204,7 +210,10 @@ def main():
     wl = workload(args.workload, args.seed)
     n, dim = wl["x"].shape
     log(f"workload {wl['desc']} precision={args.precision}")
-    eng = make_engine(wl, args.precision, args.pair_precision)
+    tuning = None
+    if args.tpp or args.tile or args.fused >= 0:
+        tuning = (args.tpp, args.tile, args.fused)
+    eng = make_engine(wl, args.precision, args.pair_precision, tuning)
     eng.step(args.warmup, "gjf")
     sync_all(eng)
     counts = eng.neighbor_counts()

@@ -30,7 +30,11 @@ template <int N, class T> __device__ __forceinline__ T dp_ipow(T x) {
   else if constexpr (N % 2 == 0) { const T h = dp_ipow<N / 2, T>(x); return h * h; }
   else return x * dp_ipow<N - 1, T>(x);
 }
-__device__ __forceinline__ float dp_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// hardware reciprocal refined by one Newton step
+__device__ __forceinline__ float dp_rcp(float x) {
+  const float y = __builtin_amdgcn_rcpf(x);
+  return fmaf(y, fmaf(-x, y, 1.0f), y);
+}
 #if DP_MODE == 1
 // mixed mode: hardware reciprocal refined by two Newton steps (within an ulp
 // for normal arguments) instead of the correctly rounded quotient

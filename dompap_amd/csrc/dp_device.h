@@ -46,8 +46,10 @@ struct DpRep {
   double w[DP_DMAX];    // cell width L/nc
   double inv_w[DP_DMAX];// nc/L
   double nl_thresh;     // s < nl_thresh  <=>  sqrt(s) < cutoff (exact, FP64)
+  u64 nlq_lo, nlq_hi;   // fixed-point r^2 (units^2) below: certainly listed; above: certainly not
   double skin_half_sq;  // (0.5 * skin)^2
   double dt, T_, tau;   // integrator parameters
+  double g1_sq, g1_ca, g1_cb;  // G-JF coefficients for mass 1 (integrator.py:17-40 order, on the host)
   i32 nhalo;            // prod(Th)
   i32 nhome;            // prod(T)
   i32 collapsed_mask;   // bit d set if nc[d] == 1
@@ -199,6 +201,25 @@ __device__ __forceinline__ double dp_normal(u64 seed, u64 step, u32 pid, int d) 
   double z0, z1;
   dp_box_muller(r, z0, z1);
   return (d & 1) ? z1 : z0;
+}
+
+// Standard normal numbers 2k and 2k + 1 of particle `pid` at step `step`
+// (one Philox block and one Box-Muller pair: dp_normal(.., 2k) and
+// dp_normal(.., 2k + 1) in one call).
+__device__ __forceinline__ void dp_normal2(u64 seed, u64 step, u32 pid, int k, double& z0, double& z1) {
+  DpU4 c;
+  c.x = pid;
+  c.y = (u32)k;
+  c.z = (u32)step;
+  c.w = (u32)(step >> 32);
+  dp_box_muller(dp_philox4x32_10(c, (u32)seed, (u32)(seed >> 32)), z0, z1);
+}
+
+// positions.py:90-98 wrap of one coordinate: shift = floor(x / L) (0 without
+// the division when 0 < x < L, where the rounded quotient is in (0, 1); zeros
+// take the division, which keeps the sign of a -0 shift).
+__device__ __forceinline__ double dp_wrap_shift(double x, double L) {
+  return (x > 0.0 && x < L) ? 0.0 : floor(x / L);
 }
 
 // ------------------------------------------------------------------------

@@ -18,15 +18,6 @@ template <int D> struct DpPack {
 // that a gated (skipped) launch costs one short wave of workgroups.
 #define DP_FOR(i, n) for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (i64)gridDim.x * blockDim.x)
 
-__global__ void k_fill_i32(i32* p, i64 n, i32 v, const DpCtl* gate) {
-  if (gate && !gate->rebuild) return;
-  DP_FOR(i, n) p[i] = v;
-}
-
-__global__ void k_fill_u64(u64* p, i64 n, u64 v) {
-  DP_FOR(i, n) p[i] = v;
-}
-
 // ---------------------------------------------------------------------------
 // Control: decide whether this step rebuilds (neighbor_list.py:183-201: the
 // previous integrator pass raised need_rebuild iff some particle's squared
@@ -121,79 +112,36 @@ __global__ void k_bin(const double* x, const i32* rep, const DpRep* reps, i64 N,
   }
 }
 
-// Exclusive scan of cell counts: cstart[c] = sum_{c' < c} count[c'];
-// cstart[C] = N.  Three passes over blocks of DP_SCAN_TILE entries.
-#define DP_SCAN_TILE 1024
-
-__device__ __forceinline__ i32 dp_block_excl_scan(i32 v, i32* sh, i32& total) {
-  // 256-thread inclusive scan in LDS (Hillis-Steele), returns exclusive
+// Single-workgroup exclusive scan of the cell counts (DP_SCAN_THREADS
+// threads, each over a contiguous run of cells): cstart[c] = sum of the
+// counts before c, cstart[C] = N.  The counts are cleared for the next build
+// as they are read.
+#define DP_SCAN_THREADS 1024
+__global__ void __launch_bounds__(DP_SCAN_THREADS) k_scan_single(i32* count, i64 C, i32* cstart, i64 N,
+                                                                  const DpCtl* ctl) {
+  if (!ctl->rebuild) return;
+  __shared__ i32 sh[DP_SCAN_THREADS];
   const int t = threadIdx.x;
-  sh[t] = v;
+  const i64 per = (C + DP_SCAN_THREADS - 1) / DP_SCAN_THREADS;
+  const i64 lo = min((i64)t * per, C), hi = min(lo + per, C);
+  i32 s = 0;
+  for (i64 c = lo; c < hi; ++c) s += count[c];
+  sh[t] = s;
   __syncthreads();
-  for (int off = 1; off < DP_BLOCK; off <<= 1) {
-    i32 add = (t >= off) ? sh[t - off] : 0;
+  for (int off = 1; off < DP_SCAN_THREADS; off <<= 1) {
+    const i32 add = (t >= off) ? sh[t - off] : 0;
     __syncthreads();
     sh[t] += add;
     __syncthreads();
   }
-  total = sh[DP_BLOCK - 1];
-  const i32 incl = sh[t];
-  __syncthreads();
-  return incl - v;
-}
-
-__global__ void k_scan_reduce(const i32* count, i64 C, i32* tile_sums, const DpCtl* ctl) {
-  if (!ctl->rebuild) return;
-  __shared__ i32 sh[DP_BLOCK];
-  for (i64 tile = blockIdx.x; tile * DP_SCAN_TILE < C; tile += gridDim.x) {
-    const i64 base = tile * DP_SCAN_TILE + (i64)threadIdx.x * 4;
-    i32 s = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (base + k < C) s += count[base + k];
-    i32 total;
-    dp_block_excl_scan(s, sh, total);
-    if (threadIdx.x == 0) tile_sums[tile] = total;
+  i32 ex = sh[t] - s;
+  for (i64 c = lo; c < hi; ++c) {
+    const i32 v = count[c];
+    cstart[c] = ex;
+    ex += v;
+    count[c] = 0;
   }
-}
-
-__global__ void k_scan_top(i32* tile_sums, i64 ntiles, const DpCtl* ctl) {
-  if (!ctl->rebuild) return;
-  __shared__ i32 sh[DP_BLOCK];
-  i32 carry = 0;
-  for (i64 base = 0; base < ntiles; base += DP_BLOCK) {
-    const i64 k = base + threadIdx.x;
-    const i32 v = (k < ntiles) ? tile_sums[k] : 0;
-    i32 total;
-    const i32 ex = dp_block_excl_scan(v, sh, total);
-    if (k < ntiles) tile_sums[k] = carry + ex;
-    carry += total;
-    __syncthreads();
-  }
-}
-
-__global__ void k_scan_down(const i32* count, i64 C, const i32* tile_offsets, i32* cstart, i64 N,
-                            const DpCtl* ctl) {
-  if (!ctl->rebuild) return;
-  __shared__ i32 sh[DP_BLOCK];
-  for (i64 tile = blockIdx.x; tile * DP_SCAN_TILE < C; tile += gridDim.x) {
-    const i64 base = tile * DP_SCAN_TILE + (i64)threadIdx.x * 4;
-    i32 v[4];
-    i32 s = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      v[k] = (base + k < C) ? count[base + k] : 0;
-      s += v[k];
-    }
-    i32 total;
-    i32 ex = dp_block_excl_scan(s, sh, total) + tile_offsets[tile];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (base + k < C) cstart[base + k] = ex;
-      ex += v[k];
-    }
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) cstart[C] = (i32)N;
+  if (t == 0) cstart[C] = (i32)N;
 }
 
 __global__ void k_scatter(const i32* cell_of, const i32* rank, const i32* cstart, i64 N, i32* tmp,
@@ -322,22 +270,58 @@ __global__ void k_make_records(i64 N, const double* x, const i32* type, const i3
 }
 
 // ---------------------------------------------------------------------------
-// Neighbour-list emit, one workgroup per tile.  The halo box's FP64 positions
-// at the last build (xref, slot order) are staged in LDS; each wavefront takes
-// one home particle at a time and sweeps its stencil rows (3 consecutive halo
-// cells along the fastest digit) 64 candidates at a time.  A candidate is kept
-// iff the FP64 distance test of neighbor_list.py:24-33 holds (evaluated as
-// s < threshold, which is exactly sqrt(s) < cutoff); ballot compaction keeps
-// the stencil order of neighbor_list.py:150-157 (order 0) / :82-86 (order 1)
-// and ascending original id inside a cell.  An entry is the candidate's index
-// in the tile's staged halo: cell occupancies are fixed between builds, so the
+// Neighbour-list emit, one workgroup per tile.  The halo box's positions at
+// the last build (xref) are staged in LDS as fixed-point records in the
+// tile's frame (offsets from the cell centre plus the halo cell's offset, as
+// in the pair kernels).  A candidate's exact squared distance in fixed-point
+// units decides it unless it lies within a few units of the list cutoff;
+// those are decided by the FP64 test of neighbor_list.py:24-33 on xref
+// (evaluated as s < threshold, exactly sqrt(s) < cutoff).  Entries keep the
+// stencil order of neighbor_list.py:150-157 (order 0) / :82-86 (order 1) and
+// ascending original id inside a cell; an entry is the candidate's index in
+// the tile's staged halo: cell occupancies are fixed between builds, so the
 // pair kernels read a neighbour's record straight from LDS with it.  Reading
 // xref makes the emit repeatable: a new tile plan re-emits the same list.
 // ---------------------------------------------------------------------------
-template <int D, typename E, bool LDS>
-__device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const DpRep& P, const double* x,
-                                                  const double* sx, int K, E* nl, i32* nlcnt, DpCtl* ctl,
-                                                  int& st_count) {
+template <int D>
+__device__ __forceinline__ int4 dp_emit_rec(const double* xref, const i32* cellpack, i64 slot, const DpRep& P,
+                                           const DpTileSmem<D>& S, int u, double qinv) {
+  const u32 pack = (u32)cellpack[slot];
+  i32 q[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const u32 c = (pack >> (DpPack<D>::BITS * d)) & DpPack<D>::MASK;
+    const i32 v = dp_quantise(xref[slot * D + d], ((double)c + 0.5) * P.w[d], P.L[d], P.halfL[d], qinv);
+    q[d] = (i32)((u32)v + dp_halo_shift<D>(S, P, u, d));
+  }
+  return make_int4(q[0], q[1], q[2], q[3]);
+}
+
+// Squared fixed-point distance (exact: |dq| < 2^31 per dimension, so the sum
+// of D <= 4 squares fits in 64 bits); CM: apply the minimum image along
+// single-cell dimensions.
+template <int D, bool CM>
+__device__ __forceinline__ u64 dp_emit_s(const int4& a, const int4& b, const DpRep& P) {
+  const i32 av[4] = {a.x, a.y, a.z, a.w};
+  const i32 bv[4] = {b.x, b.y, b.z, b.w};
+  u64 s = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    i32 dd = (i32)((u32)av[d] - (u32)bv[d]);
+    if (CM && P.nc[d] == 1) {
+      const i32 h = P.Lq[d] >> 1;
+      dd = (dd > h) ? dd - P.Lq[d] : ((dd < -h) ? dd + P.Lq[d] : dd);
+    }
+    const u32 ad = (u32)(dd < 0 ? -dd : dd);
+    s += (u64)ad * (u64)ad;
+  }
+  return s;
+}
+
+template <int D, typename E, bool LDS, bool CM>
+__device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const DpRep& P, const double* xref,
+                                                  const i32* cellpack, double qinv, const int4* srq, int K, E* nl,
+                                                  i32* nlcnt, DpCtl* ctl, int& st_count) {
   constexpr int N3 = DpPow3<D>::value;
   constexpr int NROW = N3 / 3;  // <= 27 for D <= 4: one row per lane
   const int tid = threadIdx.x;
@@ -345,11 +329,12 @@ __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const 
   const int wave = tid >> 6;
   const int nwaves = blockDim.x >> 6;
   const i64 Ks = (i64)((K + 7) & ~7);  // row stride: rows are read 8 bytes at a time
+  const u64 lo = P.nlq_lo, hi = P.nlq_hi;
   // Every particle of a home cell has the same candidates (its stencil rows,
   // in order).  Wavefront w takes particles w, w + nwaves, ... of each home
   // cell; for each chunk of 64 candidates it finds their staged indices and
-  // positions once, then tests each of its particles against the whole chunk
-  // (particle position: LDS broadcast; running row length: one lane each).
+  // records once, then tests each of its particles against the whole chunk
+  // (particle record: LDS broadcast; running row length: one lane each).
   for (int m = 0; m < P.nhome; ++m) {
     const int uh = S.home_u[m];
     if (uh < 0) continue;
@@ -360,7 +345,7 @@ __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const 
     int r_lo = 0, r_len = 0, r_u0 = 0;
     if (lane < NROW) {
       bool any = false;
-      int hi = 0;
+      int hcell = 0;
       for (int dx = 0; dx < 3; ++dx) {
         const int st = 3 * lane + dx;
         int sd[D];
@@ -372,9 +357,9 @@ __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const 
         if (!ok) continue;
         const int u = uh + S.st_off[st];
         if (!any) { any = true; r_u0 = u; r_lo = S.hbase[u]; }
-        hi = S.hbase[u + 1];
+        hcell = S.hbase[u + 1];
       }
-      r_len = any ? hi - r_lo : 0;
+      r_len = any ? hcell - r_lo : 0;
     }
     int incl = r_len;
 #pragma unroll
@@ -404,33 +389,30 @@ __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const 
         const int u0_row = __shfl(r_u0, row);
         const bool valid = c < ncand;
         const int idx = valid ? lo_row + (c - pf_row) : 0;
-        double xj[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) xj[d] = 0.0;
+        int uj = u0_row;
+        int4 qj = make_int4(0, 0, 0, 0);
         if (valid) {
-          if (LDS) {
-#pragma unroll
-            for (int d = 0; d < D; ++d) xj[d] = sx[idx * D + d];
-          } else {
-            int u = u0_row;
-            while (S.hbase[u + 1] <= idx) ++u;
-            const i64 slot_j = (i64)S.hstart[u] + (idx - S.hbase[u]);
-#pragma unroll
-            for (int d = 0; d < D; ++d) xj[d] = x[slot_j * D + d];
-          }
+          while (S.hbase[uj + 1] <= idx) ++uj;
+          qj = LDS ? srq[idx] : dp_emit_rec<D>(xref, cellpack, (i64)S.hstart[uj] + (idx - S.hbase[uj]), P, S, uj, qinv);
         }
         for (int jj = 0; jj < nj; ++jj) {
           const int k = wave + nwaves * (j0 + jj);
           const int self = self0 + k;
           const i64 slot_i = slot0 + k;
-          double ss = 0.0;
+          const int4 qi = LDS ? srq[self] : dp_emit_rec<D>(xref, cellpack, slot_i, P, S, uh, qinv);
+          const u64 s = dp_emit_s<D, CM>(qi, qj, P);
+          bool hit = valid && (idx != self) && (s < lo);
+          if (valid && idx != self && s >= lo && s <= hi) {
+            // within a few fixed-point units of the cutoff: the reference's test
+            const i64 slot_j = (i64)S.hstart[uj] + (idx - S.hbase[uj]);
+            double ss = 0.0;
 #pragma unroll
-          for (int d = 0; d < D; ++d) {
-            const double xi = LDS ? sx[self * D + d] : x[slot_i * D + d];
-            const double tt = dp_minimage_lt(dp_sub(xi, xj[d]), P.L[d], P.halfL[d]);
-            ss = (d == 0) ? dp_mul(tt, tt) : dp_add(ss, dp_mul(tt, tt));
+            for (int d = 0; d < D; ++d) {
+              const double tt = dp_minimage_lt(dp_sub(xref[slot_i * D + d], xref[slot_j * D + d]), P.L[d], P.halfL[d]);
+              ss = (d == 0) ? dp_mul(tt, tt) : dp_add(ss, dp_mul(tt, tt));
+            }
+            hit = ss < P.nl_thresh;
           }
-          const bool hit = valid && (idx != self) && (ss < P.nl_thresh);
           const u64 mask = __ballot(hit);
           const int c0 = __builtin_amdgcn_readlane(cnt, jj);
           const int pos = c0 + __popcll(mask & ((1ull << lane) - 1ull));
@@ -452,16 +434,16 @@ __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const 
 }
 
 template <int D, typename E>
-__global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile(const double* xref, const i32* cstart,
-                                                               const DpRep* reps, int R, i64 ntiles, int K,
-                                                               int cap, E* nl, i32* nlcnt, i32* tiletab,
-                                                               int tstride, DpCtl* ctl, int force) {
+__global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile(const double* xref, const i32* cellpack, double qinv,
+                                                               const i32* cstart, const DpRep* reps, int R,
+                                                               i64 ntiles, int K, int cap, E* nl, i32* nlcnt,
+                                                               i32* tiletab, int tstride, DpCtl* ctl, int force) {
   if (!force && !ctl->rebuild) return;
   // non-finite positions were binned into arbitrary cells: skip the (then
   // unbounded) emit; the error is reported after the step block
   if (ctl->nonfinite) return;
   __shared__ DpTileSmem<D> S;
-  extern __shared__ __align__(16) double sx[];  // cap * D staged positions
+  extern __shared__ __align__(16) int4 srq[];  // cap staged records
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -475,9 +457,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile(const double* xre
     if (fits) {
       for (int idx = tid; idx < S.nrec; idx += blockDim.x) {
         const int u = dp_halo_of<D>(S, P.nhalo, idx);
-        const i64 slot = S.hstart[u] + (idx - S.hbase[u]);
-#pragma unroll
-        for (int d = 0; d < D; ++d) sx[idx * D + d] = xref[slot * D + d];
+        srq[idx] = dp_emit_rec<D>(xref, cellpack, (i64)S.hstart[u] + (idx - S.hbase[u]), P, S, u, qinv);
       }
     }
     if (tid == 0) {
@@ -487,10 +467,14 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile(const double* xre
       st_halo = max(st_halo, S.nrec);
     }
     __syncthreads();
-    if (fits)
-      dp_emit_particles<D, E, true>(S, P, xref, sx, K, nl, nlcnt, ctl, st_count);
-    else
-      dp_emit_particles<D, E, false>(S, P, xref, sx, K, nl, nlcnt, ctl, st_count);
+    const bool cm = P.collapsed_mask != 0;
+    if (fits) {
+      if (cm) dp_emit_particles<D, E, true, true>(S, P, xref, cellpack, qinv, srq, K, nl, nlcnt, ctl, st_count);
+      else dp_emit_particles<D, E, true, false>(S, P, xref, cellpack, qinv, srq, K, nl, nlcnt, ctl, st_count);
+    } else {
+      if (cm) dp_emit_particles<D, E, false, true>(S, P, xref, cellpack, qinv, srq, K, nl, nlcnt, ctl, st_count);
+      else dp_emit_particles<D, E, false, false>(S, P, xref, cellpack, qinv, srq, K, nl, nlcnt, ctl, st_count);
+    }
     __syncthreads();
   }
   // statistics: one update per workgroup, spread over the slots

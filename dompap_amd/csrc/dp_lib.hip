@@ -155,7 +155,7 @@ struct dp_ctx {
   int D = 3, mode = DP_FP64, R = 1, rs = 4;
   i64 N = 0;
   int K = 0, order = DP_ORDER_X_FASTEST;
-  i64 Ctot = 0, ntiles_scan = 0;
+  i64 Ctot = 0;
   std::string err;
 
   // host mirrors of parameters
@@ -181,13 +181,14 @@ struct dp_ctx {
   double *E = nullptr, *W = nullptr, *LAP = nullptr, *xi = nullptr, *red = nullptr;
   i32 *img = nullptr, *type = nullptr, *orig = nullptr, *rep = nullptr, *cellpack = nullptr;
   i32 *cell_of = nullptr, *rank = nullptr, *tmp = nullptr, *perm = nullptr, *nlcnt = nullptr;
-  i32 *ccount = nullptr, *cstart = nullptr, *tiles = nullptr;
+  i32 *ccount = nullptr, *cstart = nullptr;
   void* nl = nullptr;
   size_t nl_bytes = 0;
   // tile / entry-format plan (refresh_reps)
   int nlw = 16, ob = 11, tpp = 8, qbits = 24, cap = 1024, cap_emit = 1024;
   // tuning (0 = automatic)
-  int tune_tpp = 0, tune_tile = 0, fused = 1;
+  int tune_tpp = 0, tune_tile = 0, tune_fused = -1;
+  int fused = 0;  // 1: fused force + integrate kernel; 0: force kernel, then integrate kernel
   int cmask = 0;  // OR of the replicas' collapsed-dimension masks
   i64 ntiles = 0;
   int tile_grid = 8;
@@ -253,7 +254,9 @@ struct dp_ctx {
 
 namespace {
 
-int grid_for(i64 n, int cap = 2048) {
+// Grid of the grid-stride pipeline kernels: at most 512 workgroups, so that
+// a launch skipped by its rebuild gate retires quickly.
+int grid_for(i64 n, int cap = 512) {
   i64 b = (n + DP_BLOCK - 1) / DP_BLOCK;
   if (b < 1) b = 1;
   if (b > cap) b = cap;
@@ -319,12 +322,14 @@ int refresh_reps(dp_ctx* ctx) {
     const double kb = std::min(n, n / vol * ball_volume(D) * std::pow(std::max(cut, 0.0), D));
     kbar_max = std::max(kbar_max, kb);
   }
+  // measured on MI355X (3D LJ, ~95 entries per row): 2 lanes per particle
+  // and the split step are fastest; the fused kernel idles lanes while one
+  // lane per dimension integrates
   int tpp = 1;
-  if (kbar_max >= 160) tpp = 16;
-  else if (kbar_max >= 48) tpp = 8;
-  else if (kbar_max >= 20) tpp = 4;
-  else if (kbar_max >= 8) tpp = 2;
+  if (kbar_max >= 160) tpp = 4;
+  else if (kbar_max >= 24) tpp = 2;
   if (ctx->tune_tpp > 0) tpp = ctx->tune_tpp;
+  ctx->fused = ctx->tune_fused >= 0 ? ctx->tune_fused : 0;
   // home particles per tile: one pass of the workgroup's lane groups
   const int target = ctx->tune_tile > 0 ? ctx->tune_tile : DP_TILE_THREADS / tpp;
   i64 c0 = 0, t0 = 0;
@@ -422,6 +427,19 @@ int refresh_reps(dp_ctx* ctx) {
     P.dt = ctx->have_integ ? ctx->dt[r] : 0.01;
     P.T_ = ctx->have_integ ? ctx->T[r] : 1.0;
     P.tau = ctx->have_integ ? ctx->tau[r] : 0.1;
+    {
+      // G-JF coefficients of a unit mass, in integrator.py:17-40's order
+      // (IEEE operations, so the device's per-particle values are identical)
+      const double m = 1.0;
+      const double alpha = m / P.tau;
+      const double bvar = 2.0 * alpha * P.T_ * P.dt;
+      const double qq = alpha * P.dt / 2.0 / m;
+      const double num = 1.0 - qq;
+      const double den = 1.0 + qq;
+      P.g1_sq = std::sqrt(bvar);
+      P.g1_ca = num / den;
+      P.g1_cb = 1.0 / den;
+    }
   }
   if (c0 >= (i64)0x7fffffff) return ctx->fail(DP_ERR_ARG, "too many cells (%lld)", (long long)c0);
   ctx->cmask = 0;
@@ -431,11 +449,30 @@ int refresh_reps(dp_ctx* ctx) {
   if (qbits > 30) qbits = 30;
   ctx->qbits = qbits;
   const double qs = std::ldexp(1.0, qbits);
-  for (auto& P : reps)
+  for (auto& P : reps) {
     for (int d = 0; d < D; ++d) {
       P.Wq[d] = (i32)std::rint(P.w[d] * qs);
       P.Lq[d] = (i32)std::min(std::rint(P.L[d] * qs), 2147483647.0);
     }
+    // emit screening: fixed-point distances differ from the exact ones by
+    // less than 1.5 units per dimension (two roundings of offsets, one of the
+    // cell width, one of the box length) -> a band of `margin` units around
+    // the cutoff is decided by the FP64 test
+    const double margin = 2.0 * std::sqrt((double)D) + 2.0;
+    const double thr = P.nl_thresh;
+    if (!(thr > 0.0)) {
+      P.nlq_lo = 0;
+      P.nlq_hi = 0;
+    } else if (!std::isfinite(thr) || std::sqrt(thr) * qs + margin > 2.0e9 * std::sqrt((double)D)) {
+      P.nlq_lo = 0;  // everything goes through the FP64 test (no fixed-point range)
+      P.nlq_hi = ~(u64)0;
+    } else {
+      const double Rq = std::sqrt(thr) * qs;
+      const double a = std::max(Rq - margin, 0.0), b = Rq + margin;
+      P.nlq_lo = (u64)std::max(std::floor(a * a) - 1.0e4, 0.0);
+      P.nlq_hi = (u64)(std::ceil(b * b) + 1.0e4);
+    }
+  }
   // entry format: the index of a neighbour in its tile's staged halo; 16
   // bits unless a halo could hold more than 2^16 particles
   (void)occ_max;
@@ -469,10 +506,10 @@ int refresh_reps(dp_ctx* ctx) {
   ctx->lds_pair = (size_t)cap * rec_bytes;
   {
     i64 ce = (i64)std::ceil(1.5 * halo_occ_max) + 64;
-    ce = std::min<i64>(ce, (i64)(48 * 1024 / (D * 8)));
+    ce = std::min<i64>(ce, (i64)(48 * 1024 / 16));
     ctx->cap_emit = (int)std::max<i64>(ce, 64);
   }
-  ctx->lds_emit = (size_t)ctx->cap_emit * D * 8;
+  ctx->lds_emit = (size_t)ctx->cap_emit * 16;  // one int4 record per staged particle
   ctx->ntiles = t0;
   {
     i64 g = std::min<i64>(t0, 4096);
@@ -482,10 +519,9 @@ int refresh_reps(dp_ctx* ctx) {
   ctx->reps_h = reps;
   if (c0 != ctx->Ctot) {
     ctx->Ctot = c0;
-    ctx->ntiles_scan = (c0 + DP_SCAN_TILE - 1) / DP_SCAN_TILE;
     CHK(dalloc(ctx, ctx->ccount, (size_t)c0));
+    HIPCHK(hipMemsetAsync(ctx->ccount, 0, (size_t)c0 * 4, ctx->st));
     CHK(dalloc(ctx, ctx->cstart, (size_t)c0 + 1));
-    CHK(dalloc(ctx, ctx->tiles, (size_t)ctx->ntiles_scan + 1));
     release_graph(ctx);
   }
   // tile tables: one fixed stride for all tiles
@@ -679,11 +715,11 @@ template <int D>
 int launch_emit_t(dp_ctx* ctx, int force) {
   if (ctx->nlw == 16)
     k_emit_tile<D, u16><<<ctx->tile_grid, DP_TILE_THREADS, ctx->lds_emit, ctx->st>>>(
-        ctx->xref, ctx->cstart, ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, ctx->cap_emit, (u16*)ctx->nl, ctx->nlcnt,
+        ctx->xref, ctx->cellpack, std::ldexp(1.0, ctx->qbits), ctx->cstart, ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, ctx->cap_emit, (u16*)ctx->nl, ctx->nlcnt,
         ctx->tiletab, ctx->tstride, ctx->ctl_d, force);
   else
     k_emit_tile<D, u32><<<ctx->tile_grid, DP_TILE_THREADS, ctx->lds_emit, ctx->st>>>(
-        ctx->xref, ctx->cstart, ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, ctx->cap_emit, (u32*)ctx->nl, ctx->nlcnt,
+        ctx->xref, ctx->cellpack, std::ldexp(1.0, ctx->qbits), ctx->cstart, ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, ctx->cap_emit, (u32*)ctx->nl, ctx->nlcnt,
         ctx->tiletab, ctx->tstride, ctx->ctl_d, force);
   HIPCHK(hipGetLastError());
   return DP_OK;
@@ -706,14 +742,10 @@ int launch_rebuild_t(dp_ctx* ctx) {
   const i64 N = ctx->N;
   const int gN = grid_for(N);
   hipStream_t st = ctx->st;
-  k_fill_i32<<<grid_for(ctx->Ctot), DP_BLOCK, 0, st>>>(ctx->ccount, ctx->Ctot, 0, ctx->ctl_d);
+  // the cell counts are zero here: cleared at allocation and by the scan
   k_bin<D><<<gN, DP_BLOCK, 0, st>>>(ctx->x[cur], ctx->rep, ctx->reps_d, N, ctx->cell_of, ctx->rank, ctx->ccount,
                                      ctx->ctl_d);
-  k_scan_reduce<<<grid_for(ctx->ntiles_scan * DP_BLOCK), DP_BLOCK, 0, st>>>(ctx->ccount, ctx->Ctot, ctx->tiles,
-                                                                          ctx->ctl_d);
-  k_scan_top<<<1, DP_BLOCK, 0, st>>>(ctx->tiles, ctx->ntiles_scan, ctx->ctl_d);
-  k_scan_down<<<grid_for(ctx->ntiles_scan * DP_BLOCK), DP_BLOCK, 0, st>>>(ctx->ccount, ctx->Ctot, ctx->tiles, ctx->cstart,
-                                                                    N, ctx->ctl_d);
+  k_scan_single<<<1, DP_SCAN_THREADS, 0, st>>>(ctx->ccount, ctx->Ctot, ctx->cstart, N, ctx->ctl_d);
   k_scatter<<<gN, DP_BLOCK, 0, st>>>(ctx->cell_of, ctx->rank, ctx->cstart, N, ctx->tmp, ctx->ctl_d);
   k_cellsort<<<gN, DP_BLOCK, 0, st>>>(ctx->tmp, ctx->cell_of, ctx->cstart, ctx->orig, N, ctx->perm, ctx->ctl_d);
   k_permute<D><<<gN, DP_BLOCK, 0, st>>>(ctx->perm, N, ctx->x[cur], ctx->v, ctx->beta, ctx->mass, ctx->img,
@@ -1130,7 +1162,7 @@ DP_API int dp_destroy(dp_ctx* ctx) {
   dfree(ctx->v); dfree(ctx->beta); dfree(ctx->mass); dfree(ctx->xref); dfree(ctx->F); dfree(ctx->E);
   dfree(ctx->W); dfree(ctx->LAP); dfree(ctx->xi); dfree(ctx->img); dfree(ctx->type); dfree(ctx->orig);
   dfree(ctx->rep); dfree(ctx->cellpack); dfree(ctx->cell_of); dfree(ctx->rank); dfree(ctx->tmp);
-  dfree(ctx->perm); dfree(ctx->nlcnt); dfree(ctx->ccount); dfree(ctx->cstart); dfree(ctx->tiles);
+  dfree(ctx->perm); dfree(ctx->nlcnt); dfree(ctx->ccount); dfree(ctx->cstart);
   dfree(ctx->nl); dfree(ctx->tiletab); dfree(ctx->scratch); dfree(ctx->stage);
   if (ctx->ctl_h) (void)hipHostFree(ctx->ctl_h);
   if (ctx->st) (void)hipStreamDestroy(ctx->st);
@@ -1448,7 +1480,7 @@ DP_API int dp_set_tuning(dp_ctx* ctx, int threads_per_particle, int tile_particl
   if (tile_particles < 0) return ctx->fail(DP_ERR_ARG, "tile_particles must be >= 0");
   ctx->tune_tpp = threads_per_particle;
   ctx->tune_tile = tile_particles;
-  ctx->fused = fused ? 1 : 0;
+  ctx->tune_fused = fused < 0 ? -1 : (fused ? 1 : 0);
   ctx->reps_dirty = true;
   release_graph(ctx);
   return DP_OK;

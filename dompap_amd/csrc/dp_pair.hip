@@ -131,6 +131,49 @@ __device__ __forceinline__ void dp_for_chunks(const dp_ent_t* row, int cnt, int 
   }
 }
 
+// integrator.py:17-40 (G-JF) coefficients of mass m, numpy's left-to-right
+// evaluation order; a unit mass takes the host's values (same operations).
+__device__ __forceinline__ void dp_gjf_coeffs(const DpRep& P, double m, double& sq, double& ca, double& cb) {
+#pragma clang fp contract(off)
+  if (m == 1.0) {
+    sq = P.g1_sq;
+    ca = P.g1_ca;
+    cb = P.g1_cb;
+  } else {
+    const double dt = P.dt;
+    const double alpha = m / P.tau;
+    const double bvar = 2.0 * alpha * P.T_ * dt;
+    sq = sqrt(bvar);
+    const double qq = alpha * dt / 2.0 / m;
+    const double num = 1.0 - qq;
+    const double den = 1.0 + qq;
+    ca = num / den;
+    cb = 1.0 / den;
+  }
+}
+
+// One dimension of the G-JF velocity update,
+//   v1 = ca * v0 + cb * dt * f / m + cb * (b0 + nb) / 2 / m,
+// evaluated left to right; dividing by a unit mass is exact, so m == 1 skips it.
+__device__ __forceinline__ double dp_gjf_v(double ca, double cb, double dt, double m, double v0, double f,
+                                           double b0, double nb) {
+#pragma clang fp contract(off)
+  double t1 = cb * dt * f;
+  double t2 = cb * (b0 + nb) / 2.0;
+  if (m != 1.0) {
+    t1 = t1 / m;
+    t2 = t2 / m;
+  }
+  return ca * v0 + t1 + t2;
+}
+
+// integrator.py:5-14 (leap-frog) velocity update v + dt * f / m.
+__device__ __forceinline__ double dp_lf_v(double dt, double m, double v0, double f) {
+#pragma clang fp contract(off)
+  const double t = dt * f;
+  return v0 + ((m != 1.0) ? t / m : t);
+}
+
 // XCD-aware tile order: workgroup b runs on XCD b % 8; the tiles of XCD k are
 // the contiguous chunk [k * chunk, (k + 1) * chunk), shared by that XCD's
 // workgroups in turn.
@@ -249,15 +292,13 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
           rj[c] = dp_halo_rec(a, S, P, u, (i64)S.hstart[u] + (ic - S.hbase[u]));
         }
       }
-      i32 dq[DP_EPL][DP_D];
-      float dqf[DP_EPL][DP_D];
-      float s2[DP_EPL];
+      double dq[DP_EPL][DP_D];  // exact: 32-bit fixed-point differences
+      float s2[DP_EPL];         // r^2 in fixed-point units^2, rounded once
       int tt[DP_EPL];
       bool in[DP_EPL];
       bool near = false;
 #pragma unroll
       for (int c = 0; c < DP_EPL; ++c) {
-        s2[c] = 0.f;
 #pragma unroll
         for (int d = 0; d < DP_D; ++d) {
           i32 dd = (i32)((u32)ri.v[d] - (u32)rj[c].v[d]);
@@ -267,10 +308,12 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
             const i32 h = P.Lq[d] >> 1;
             dd = (dd > h) ? dd - P.Lq[d] : ((dd < -h) ? dd + P.Lq[d] : dd);
           }
-          dq[c][d] = dd;
-          dqf[c][d] = (float)dd;
-          s2[c] = fmaf(dqf[c][d], dqf[c][d], s2[c]);
+          dq[c][d] = (double)dd;
         }
+        double sd = dq[c][0] * dq[c][0];
+#pragma unroll
+        for (int d = 1; d < DP_D; ++d) sd = fma(dq[c][d], dq[c][d], sd);
+        s2[c] = (float)sd;
         tt[c] = dp_tt(ti, (DP_NTYPES > 1) ? rj[c].v[DP_D] : 0);
         const DpPC& pc = DP_PC(tt[c]);
         in[c] = (c < nvalid) && (s2[c] < pc.ghi);  // else certainly beyond the cut
@@ -293,18 +336,13 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
           in[c] = !(s > DP_PC(tt[c]).cut);
         }
       }
-#if DP_PAIRPREC == 32
-      float fc[DP_D];  // FP32 partial sums over the chunk, added to the FP64 totals below
-#pragma unroll
-      for (int d = 0; d < DP_D; ++d) fc[d] = 0.f;
-#endif
 #pragma unroll
       for (int c = 0; c < DP_EPL; ++c) {
 #if DP_PAIRPREC == 64
         const DpPC& pc = DP_PC(tt[c]);
-        double s64 = 0.0;
+        double s64 = dq[c][0] * dq[c][0];
 #pragma unroll
-        for (int d = 0; d < DP_D; ++d) s64 = fma((double)dq[c][d], (double)dq[c][d], s64);
+        for (int d = 1; d < DP_D; ++d) s64 = fma(dq[c][d], dq[c][d], s64);
         // entries off the list (padding, beyond the cut) evaluate at s = 1
         const double sx = in[c] ? s64 * pc.ks : 1.0;
 #if DP_EVEN
@@ -315,7 +353,7 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
 #endif
         const double fs = in[c] ? pc.ke * g : 0.0;  // eps f(x)/r per fixed-point unit
 #pragma unroll
-        for (int d = 0; d < DP_D; ++d) f[d] = fma(fs, (double)dq[c][d], f[d]);
+        for (int d = 0; d < DP_D; ++d) f[d] = fma(fs, dq[c][d], f[d]);
         if (KIND == 1 && in[c]) {
 #if DP_EVEN
           e += pc.eps * (dp_pot_us<double>(sx) - T.u_rc);
@@ -334,9 +372,10 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
         const float xrf = sqrtf(sxf);
         const float g = dp_pot_f<float>(xrf) * dp_rcp(xrf);
 #endif
-        const float fs = in[c] ? pc.kef * g : 0.0f;
+        // FP32 pair term, accumulated in FP64 with the exact displacement
+        const double fs = in[c] ? (double)(pc.kef * g) : 0.0;
 #pragma unroll
-        for (int d = 0; d < DP_D; ++d) fc[d] = fmaf(fs, dqf[c][d], fc[d]);
+        for (int d = 0; d < DP_D; ++d) f[d] = fma(fs, dq[c][d], f[d]);
         if (KIND == 1 && in[c]) {
 #if DP_EVEN
           e += (double)(pc.epsf * dp_pot_us<float>(sxf)) - (double)pc.epsf * T.u_rc;
@@ -349,10 +388,6 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
         }
 #endif
       }
-#if DP_PAIRPREC == 32
-#pragma unroll
-      for (int d = 0; d < DP_D; ++d) f[d] += (double)fc[d];
-#endif
     };
     dp_for_chunks(row, cnt, sub, chunk);
 #else
@@ -450,15 +485,8 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
           if (d < DP_D) z_own[j] = a.xi ? a.xi[(i64)pid * DP_D + d] : dp_normal(a.seed, a.ctl->step_index, pid, d);
         }
       }
-      // integrator.py:17-40 (G-JF) coefficients, numpy's left-to-right order
-      const double alpha = m / P.tau;
-      const double bvar = 2.0 * alpha * P.T_ * dt;
-      const double sq = sqrt(bvar);
-      const double qq = alpha * dt / 2.0 / m;
-      const double num = 1.0 - qq;
-      const double den = 1.0 + qq;
-      const double ca = num / den;
-      const double cb = 1.0 / den;
+      double sq = 0.0, ca = 0.0, cb = 0.0;
+      if (KIND == 2) dp_gjf_coeffs(P, m, sq, ca, cb);
 #if DP_MODE == 1
       const u32 pack = (u32)a.cellpack[ii];
 #endif
@@ -471,23 +499,23 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
         double xn;
         if (KIND == 2) {
           const double nb = sq * z_own[d / DP_TPP];
-          const double v0 = a.v[ii * DP_D + d];
-          const double b0 = a.beta[ii * DP_D + d];
-          const double v1 = ca * v0 + cb * dt * f[d] / m + cb * (b0 + nb) / 2.0 / m;
+          const double v1 = dp_gjf_v(ca, cb, dt, m, a.v[ii * DP_D + d], f[d], a.beta[ii * DP_D + d], nb);
           xn = a.x[ii * DP_D + d] + v1 * dt;
           a.v[ii * DP_D + d] = v1;
           a.beta[ii * DP_D + d] = nb;
         } else {
           // integrator.py:5-14 (leap-frog)
-          const double v1 = a.v[ii * DP_D + d] + dt * f[d] / m;
+          const double v1 = dp_lf_v(dt, m, a.v[ii * DP_D + d], f[d]);
           xn = a.x[ii * DP_D + d] + dt * v1;
           a.v[ii * DP_D + d] = v1;
         }
         // positions.py:90-98 wrap, then neighbor_list.py:183-201 displacement
         if (a.wrap) {
-          const double sh = floor(xn / P.L[d]);
-          a.img[ii * DP_D + d] += (i32)sh;
-          xn = xn - sh * P.L[d];
+          const double sh = dp_wrap_shift(xn, P.L[d]);
+          if (!(xn > 0.0 && xn < P.L[d])) {
+            a.img[ii * DP_D + d] += (i32)sh;
+            xn = xn - sh * P.L[d];
+          }
         }
         a.x_out[ii * DP_D + d] = xn;
         tq_own[d / DP_TPP] = dp_minimage_gt(dp_sub(xn, a.xref[ii * DP_D + d]), P.L[d], P.halfL[d]);
@@ -594,21 +622,26 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
     const u32 pid = (u32)a.orig[ii];
     double xn[DP_D];
     if (INTEG == 1) {
-      const double alpha = m / P.tau;
-      const double bvar = 2.0 * alpha * P.T_ * dt;
-      const double sq = sqrt(bvar);
-      const double qq = alpha * dt / 2.0 / m;
-      const double num = 1.0 - qq;
-      const double den = 1.0 + qq;
-      const double ca = num / den;
-      const double cb = 1.0 / den;
+      double sq, ca, cb;
+      dp_gjf_coeffs(P, m, sq, ca, cb);
+      // normal numbers in pairs: one Philox block and Box-Muller per two dimensions
+      double z[DP_D];
+      if (a.xi) {
+#pragma unroll
+        for (int d = 0; d < DP_D; ++d) z[d] = a.xi[(i64)pid * DP_D + d];
+      } else {
+#pragma unroll
+        for (int k = 0; 2 * k < DP_D; ++k) {
+          double z0, z1;
+          dp_normal2(a.seed, step, pid, k, z0, z1);
+          z[2 * k] = z0;
+          if (2 * k + 1 < DP_D) z[2 * k + 1] = z1;
+        }
+      }
 #pragma unroll
       for (int d = 0; d < DP_D; ++d) {
-        const double z = a.xi ? a.xi[(i64)pid * DP_D + d] : dp_normal(a.seed, step, pid, d);
-        const double nb = sq * z;
-        const double v0 = a.v[ii * DP_D + d];
-        const double b0 = a.beta[ii * DP_D + d];
-        const double v1 = ca * v0 + cb * dt * a.F[ii * DP_D + d] / m + cb * (b0 + nb) / 2.0 / m;
+        const double nb = sq * z[d];
+        const double v1 = dp_gjf_v(ca, cb, dt, m, a.v[ii * DP_D + d], a.F[ii * DP_D + d], a.beta[ii * DP_D + d], nb);
         xn[d] = a.x[ii * DP_D + d] + v1 * dt;
         a.v[ii * DP_D + d] = v1;
         a.beta[ii * DP_D + d] = nb;
@@ -616,7 +649,7 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
     } else {
 #pragma unroll
       for (int d = 0; d < DP_D; ++d) {
-        const double v1 = a.v[ii * DP_D + d] + dt * a.F[ii * DP_D + d] / m;
+        const double v1 = dp_lf_v(dt, m, a.v[ii * DP_D + d], a.F[ii * DP_D + d]);
         xn[d] = a.x[ii * DP_D + d] + dt * v1;
         a.v[ii * DP_D + d] = v1;
       }
@@ -624,9 +657,11 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
 #pragma unroll
     for (int d = 0; d < DP_D; ++d) {
       if (a.wrap) {
-        const double sh = floor(xn[d] / P.L[d]);
-        a.img[ii * DP_D + d] += (i32)sh;
-        xn[d] = xn[d] - sh * P.L[d];
+        const double sh = dp_wrap_shift(xn[d], P.L[d]);
+        if (!(xn[d] > 0.0 && xn[d] < P.L[d])) {
+          a.img[ii * DP_D + d] += (i32)sh;
+          xn[d] = xn[d] - sh * P.L[d];
+        }
       }
       a.x_out[ii * DP_D + d] = xn[d];
       if (a.wrap) {

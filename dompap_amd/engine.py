@@ -124,8 +124,11 @@ class Engine:
     def set_seed(self, seed: int, step_counter: int = 0) -> None:
         self._chk(self.lib.dp_set_seed(self.ctx, int(seed) & (2**64 - 1), int(step_counter)))
 
-    def set_tuning(self, threads_per_particle: int = 0, tile_particles: int = 0, fused: bool = True) -> None:
-        self._chk(self.lib.dp_set_tuning(self.ctx, int(threads_per_particle), int(tile_particles), int(bool(fused))))
+    def set_tuning(self, threads_per_particle: int = 0, tile_particles: int = 0, fused: int = -1) -> None:
+        """Kernel shape: lanes per particle, home particles per tile (0 =
+        planner's choice for both) and fused (1) / split (0) step kernels
+        (-1 = planner's choice)."""
+        self._chk(self.lib.dp_set_tuning(self.ctx, int(threads_per_particle), int(tile_particles), int(fused)))
 
     PLAN_KEYS = ("threads_per_particle", "entry_bits", "offset_bits", "fixed_point_bits", "lds_records",
                  "lds_records_emit", "tiles", "tile_grid", "fused", "home_cells_per_tile", "halo_cells_per_tile",

@@ -133,7 +133,8 @@ int dp_integrate(dp_ctx* ctx, const double* forces, int integrator, const double
 /* Kernel tuning (no counterpart in the reference's C surface; the GPU
  * autotuner replacing dompap/tools/autotune.py:4-178 drives it): lanes per particle
  * (0 = automatic, else 1/2/4/8/16), home particles per tile (0 = automatic),
- * fused force+integrate kernel (1) or force kernel then integrate kernel (0). */
+ * fused force+integrate kernel (1), force kernel then integrate kernel (0) or
+ * automatic (-1). */
 int dp_set_tuning(dp_ctx* ctx, int threads_per_particle, int tile_particles, int fused);
 /* Current plan: tpp, entry bits, offset bits, fixed-point bits, LDS record
  * capacity (pair, emit), tiles, tile grid, fused, home / halo cells per tile
