@@ -100,7 +100,10 @@ def sync_all(eng):
         pass
 
 
-def timed_run(eng, steps, timing=True):
+def timed_run(eng, steps, timing=True, breakdown=False):
+    """Wall time of `steps` G-JF steps (device work bracketed by stream
+    synchronisations), the HIP-event time of the step kernels, launches and
+    rebuilds; with breakdown, also (force-kernel ms, integrate-kernel ms)."""
     upd0, st0, _ = eng.stats()
     eng.set_kernel_timing(timing)
     sync_all(eng)
@@ -109,8 +112,11 @@ def timed_run(eng, steps, timing=True):
     sync_all(eng)
     t1 = time.perf_counter()
     ms, launches = eng.kernel_time()
+    split = eng.kernel_breakdown()
     eng.set_kernel_timing(False)
     upd1, _, _ = eng.stats()
+    if breakdown:
+        return t1 - t0, ms, launches, int(upd1[0] - upd0[0]), split
     return t1 - t0, ms, launches, int(upd1[0] - upd0[0])
 
 
@@ -180,6 +186,35 @@ def cpu_baseline(wl, x, rebuild_interval, budget_s=20.0):
                       f"{rebuild_interval:.1f} steps as measured on the GPU"}, F0
 
 
+def copy_calibration():
+    """A plain copy of known size: 5 copies of 256 MiB between device buffers.
+    FETCH_SIZE / WRITE_SIZE measured on it give the counters' scale factors."""
+    import torch
+    x = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    x.uniform_()
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_per_copy": 256 * 1024 * 1024, "copies": 5}}), flush=True)
+
+
+def measured_traffic(workload_desc, plan, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (profiles/traffic.json), if it was measured on this workload and
+    tile plan; else None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for e in data.get("entries", []):
+        if e.get("workload") == workload_desc and e.get("kernel") == kernel and e.get("plan") == plan:
+            return e.get("bytes_per_launch")
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,7 +233,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-nve", action="store_true", help="skip the NVE drift segment")
     ap.add_argument("--nve-steps", type=int, default=2000)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device copy 5 times (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
@@ -219,20 +259,33 @@ def main():
     counts = eng.neighbor_counts()
     kbar = float(counts.mean())
     log(f"warmup {args.warmup} done, mean list length {kbar:.1f}")
-    # timed region: direct launches with HIP events around every step kernel
-    dt, ms, launches, rebuilds = timed_run(eng, args.steps, timing=True)
+    plan = eng.plan()
+    # timed region: direct launches with HIP events around the step kernels
+    dt, ms, launches, rebuilds, (f_ms, i_ms) = timed_run(eng, args.steps, timing=True, breakdown=True)
     value = n * args.steps / dt
     # the same number of steps replayed from captured hipGraph blocks (no events)
     dtg, _, _, _ = timed_run(eng, args.steps, timing=False)
+    nl_ = max(launches, 1)
     log(f"timed {args.steps} steps: {dt:.3f}s -> {value:.4e} particle-steps/s; rebuilds={rebuilds}; "
-        f"step kernel {ms / max(launches, 1) * 1e3:.1f} us/launch over {launches} launches")
-    # algorithmic bytes per particle-step (force + integrate), mixed / FP64
-    if args.precision == "mixed":
-        b_pp = 4 * kbar + dim * 4 + dim * 8 + 76 * dim
-    else:
-        b_pp = 4 * kbar + dim * 8 + dim * 8 + 72 * dim
-    avg_launch_s = ms / 1e3 / max(launches, 1)
-    achieved = n * b_pp / avg_launch_s / 1e9
+        f"force kernel {f_ms / nl_ * 1e3:.1f} us, integrate kernel {i_ms / nl_ * 1e3:.1f} us per step")
+    # Algorithmic HBM bytes per particle and launch.  Force kernel: the
+    # particle's neighbour row (entry bytes x mean row length), its staged
+    # record (mixed: RS ints; FP64: D doubles), its row length, its force
+    # (D doubles) and its share of the tile tables.  Integrate kernel: read
+    # x, v, beta, F, xref (5 D doubles), mass, cellpack, orig, rep; write x,
+    # v, beta (3 D doubles); mixed also rewrites its record (RS ints).
+    ent = plan["entry_bits"] // 8
+    rs = 2
+    while rs < dim + 1:
+        rs *= 2
+    home = max(n / max(plan["tiles"], 1), 1.0)
+    tab_pp = (8 + 3 * plan["halo_cells_per_tile"] + 2 * plan["home_cells_per_tile"] + 2) * 4 / home
+    rec = rs * 4 if args.precision == "mixed" else dim * 8
+    b_force = ent * kbar + rec + 4 + 8 * dim + tab_pp
+    b_integ = 8 * dim * 5 + 8 + 4 + 4 + 4 + 8 * dim * 3 + (2 * rs * 4 if args.precision == "mixed" else 0)
+    force_s = f_ms / 1e3 / nl_
+    integ_s = i_ms / 1e3 / nl_
+    achieved = n * b_force / force_s / 1e9
     out = {
         "metric": "particle-steps/s on 1 MI355X, mixed vs FP64 GPU, at stated force error",
         "value": value,
@@ -244,15 +297,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if args.precision == "mixed" else "f64",
+        # mixed: int32 fixed-point displacements and screening, pair terms in
+        # FP64 (or FP32), FP64 accumulation and integration
+        "dtype": (("i32+f64" if args.pair_precision == "fp64" else "i32+f32+f64")
+                  if args.precision == "mixed" else "f64"),
         "data": "synthetic (seeded lattice + Gaussian velocities)",
         "config": {"workload": wl["desc"] + f" {args.precision}", "n_particles": n, "dim": dim,
                    "precision": args.precision, "pair_precision": args.pair_precision, "seed": args.seed},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "dp_step_gjf (fused force + G-JF integrate)",
-                     "bytes_per_particle": b_pp, "mean_list_length": kbar,
-                     "avg_launch_us": avg_launch_s * 1e6},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(wl["desc"] + f" {args.precision}", plan,
+                                                 "dp_tile_force" if not plan["fused"] else "dp_tile_gjf"),
+                     "kernel": "dp_tile_force" if not plan["fused"] else "dp_tile_gjf",
+                     "bytes_per_particle": b_force, "mean_list_length": kbar,
+                     "avg_launch_us": force_s * 1e6,
+                     "integrate_kernel": {"kernel": "dp_integrate_gjf", "avg_launch_us": integ_s * 1e6,
+                                          "bytes_per_particle": b_integ,
+                                          "achieved_GBs": n * b_integ / integ_s / 1e9 if integ_s > 0 else None}},
+        "plan": plan,
         "graph_replay_value": n * args.steps / dtg,
         "rebuilds_in_timed_region": rebuilds,
         "rebuild_interval": args.steps / max(rebuilds, 1),

@@ -149,6 +149,12 @@ int compile_cached(const std::string& src, std::vector<char>& code, std::string&
 
 }  // namespace
 
+// Timing events of one step: before the force kernel, between the force and
+// integrate kernels, after the integrate kernel.
+struct DpEv3 {
+  hipEvent_t e[3];
+};
+
 struct dp_ctx {
   int dev = 0;
   hipStream_t st = nullptr;
@@ -224,9 +230,9 @@ struct dp_ctx {
 
   // kernel timing
   bool timing = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_free;
-  double time_ms = 0.0;
+  std::vector<DpEv3> ev_pending;
+  std::vector<DpEv3> ev_free;
+  double time_ms = 0.0, time_force_ms = 0.0, time_integ_ms = 0.0;
   i64 time_launches = 0;
 
   int fail(int code, const char* fmt, ...) {
@@ -871,42 +877,44 @@ int pre_step_checks(dp_ctx* ctx) {
   return DP_OK;
 }
 
-int get_event_pair(dp_ctx* ctx, std::pair<hipEvent_t, hipEvent_t>& ev) {
+int get_events(dp_ctx* ctx, DpEv3& ev) {
   if (!ctx->ev_free.empty()) {
     ev = ctx->ev_free.back();
     ctx->ev_free.pop_back();
     return DP_OK;
   }
-  HIPCHK(hipEventCreate(&ev.first));
-  HIPCHK(hipEventCreate(&ev.second));
+  for (auto& e : ev.e) HIPCHK(hipEventCreate(&e));
   return DP_OK;
 }
 
-// One step: control, gated rebuild, fused force+integrate kernel.  With
-// `ev`, the step kernel is bracketed by the two events (also inside a
-// captured graph, where they become event-record nodes).
-int launch_one_step(dp_ctx* ctx, int integ, const double* xi_dev, const std::pair<hipEvent_t, hipEvent_t>* ev) {
+// One step: control, gated rebuild, then the force kernel and the integrate
+// kernel (or the fused kernel).  With `ev`, e[0] / e[1] / e[2] are recorded
+// before the force kernel, between the two and after the integrate kernel
+// (e[1] == e[2] for the fused kernel).
+int launch_one_step(dp_ctx* ctx, int integ, const double* xi_dev, const DpEv3* ev) {
   CHK(launch_prologue(ctx, 1));
   CHK(launch_rebuild(ctx));
   DpPairArgs a = pair_args(ctx, ctx->cur);
   a.xi = xi_dev;
-  if (ev) HIPCHK(hipEventRecord(ev->first, ctx->st));
+  if (ev) HIPCHK(hipEventRecord(ev->e[0], ctx->st));
   if (ctx->fused) {
     CHK(launch_pair(ctx, (integ == DP_INTEGRATOR_GJF) ? ctx->f_gjf : ctx->f_lf, a));
+    if (ev) HIPCHK(hipEventRecord(ev->e[1], ctx->st));
   } else {
     a.F = ctx->F;
     CHK(launch_pair(ctx, ctx->f_force, a));
+    if (ev) HIPCHK(hipEventRecord(ev->e[1], ctx->st));
     CHK(launch_flat(ctx, (integ == DP_INTEGRATOR_GJF) ? ctx->f_int_gjf : ctx->f_int_lf, a));
   }
-  if (ev) HIPCHK(hipEventRecord(ev->second, ctx->st));
+  if (ev) HIPCHK(hipEventRecord(ev->e[2], ctx->st));
   ctx->cur = 1 - ctx->cur;
   return DP_OK;
 }
 
 int launch_one_step_timed(dp_ctx* ctx, int integ, const double* xi_dev) {
   if (!ctx->timing) return launch_one_step(ctx, integ, xi_dev, nullptr);
-  std::pair<hipEvent_t, hipEvent_t> ev;
-  CHK(get_event_pair(ctx, ev));
+  DpEv3 ev;
+  CHK(get_events(ctx, ev));
   CHK(launch_one_step(ctx, integ, xi_dev, &ev));
   ctx->ev_pending.push_back(ev);
   return DP_OK;
@@ -916,9 +924,12 @@ int resolve_timing(dp_ctx* ctx) {
   if (ctx->ev_pending.empty()) return DP_OK;
   HIPCHK(hipStreamSynchronize(ctx->st));
   for (auto& ev : ctx->ev_pending) {
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, ev.first, ev.second));
+    float ms = 0.f, m1 = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ev.e[0], ev.e[2]));
+    HIPCHK(hipEventElapsedTime(&m1, ev.e[0], ev.e[1]));
     ctx->time_ms += ms;
+    ctx->time_force_ms += m1;
+    ctx->time_integ_ms += ms - m1;
     ctx->time_launches += 1;
     ctx->ev_free.push_back(ev);
   }
@@ -1154,8 +1165,10 @@ DP_API int dp_destroy(dp_ctx* ctx) {
   (void)hipSetDevice(ctx->dev);
   if (ctx->st) (void)hipStreamSynchronize(ctx->st);
   release_graph(ctx);
-  for (auto& ev : ctx->ev_pending) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
-  for (auto& ev : ctx->ev_free) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+  for (auto& ev : ctx->ev_pending)
+    for (auto& e : ev.e) (void)hipEventDestroy(e);
+  for (auto& ev : ctx->ev_free)
+    for (auto& e : ev.e) (void)hipEventDestroy(e);
   if (ctx->mod) (void)hipModuleUnload(ctx->mod);
   dfree(ctx->reps_d); dfree(ctx->tab_d); dfree(ctx->ctl_d); dfree(ctx->nlupd_d);
   dfree(ctx->red); dfree(ctx->x[0]); dfree(ctx->x[1]); dfree(ctx->xq[0]); dfree(ctx->xq[1]);
@@ -1585,6 +1598,8 @@ DP_API int dp_set_kernel_timing(dp_ctx* ctx, int enable) {
   CHK(resolve_timing(ctx));
   ctx->timing = enable != 0;
   ctx->time_ms = 0.0;
+  ctx->time_force_ms = 0.0;
+  ctx->time_integ_ms = 0.0;
   ctx->time_launches = 0;
   return DP_OK;
 }
@@ -1595,6 +1610,15 @@ DP_API int dp_get_kernel_time(dp_ctx* ctx, double* total_ms, int64_t* launches) 
   CHK(resolve_timing(ctx));
   if (total_ms) *total_ms = ctx->time_ms;
   if (launches) *launches = ctx->time_launches;
+  return DP_OK;
+}
+
+DP_API int dp_get_kernel_breakdown(dp_ctx* ctx, double* force_ms, double* integrate_ms) {
+  if (!ctx) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  CHK(resolve_timing(ctx));
+  if (force_ms) *force_ms = ctx->time_force_ms;
+  if (integrate_ms) *integrate_ms = ctx->time_integ_ms;
   return DP_OK;
 }
 

@@ -233,6 +233,13 @@ class Engine:
         self._chk(self.lib.dp_get_kernel_time(self.ctx, ctypes.byref(ms), ctypes.byref(n)))
         return float(ms.value), int(n.value)
 
+    def kernel_breakdown(self):
+        """(force-kernel ms, integrate-kernel ms) summed over the timed steps."""
+        f = ctypes.c_double()
+        g = ctypes.c_double()
+        self._chk(self.lib.dp_get_kernel_breakdown(self.ctx, ctypes.byref(f), ctypes.byref(g)))
+        return float(f.value), float(g.value)
+
     def stream(self) -> int:
         s = ctypes.c_void_p()
         self._chk(self.lib.dp_get_stream(self.ctx, ctypes.byref(s)))

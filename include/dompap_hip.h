@@ -183,6 +183,10 @@ int dp_compile_check_tuned(int dim, int precision, int ntypes, int even, const c
 int dp_compile_code(int dim, int precision, int ntypes, int even, const char* device_src, int pair_precision,
                     char* out, int64_t out_len, int64_t* code_size);
 
+/* Split of the timed step-kernel time: force (or fused) kernel and integrate
+ * kernel, each summed over the timed steps (no counterpart in the reference). */
+int dp_get_kernel_breakdown(dp_ctx* ctx, double* force_ms, double* integrate_ms);
+
 /* Block until all work on the context stream is done. */
 int dp_sync(dp_ctx* ctx);
 
