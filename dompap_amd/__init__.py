@@ -7,5 +7,6 @@ kernels for gfx950 (libdompap_hip.so, C-ABI in include/dompap_hip.h); any
 SymPy pair potential is compiled to device code with hiprtc.
 """
 from .Simulation import Simulation  # noqa: F401
+from .batch import ReplicaBatch  # noqa: F401
 
 __version__ = "0.1.0"

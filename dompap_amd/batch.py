@@ -1,0 +1,156 @@
+"""Batched replicas: many independent systems stepped together on one GPU.
+
+The reference runs one ``Simulation`` per process.  ``ReplicaBatch`` takes a
+list of ``Simulation`` objects of one dimension and one pair potential and
+runs them as the replicas of one device context: one launch per kernel for
+all of them, K-step hipGraph blocks, per-replica boxes, cutoffs, skins,
+time steps, temperatures and damping times.  Each replica's noise is keyed by
+its own seed and the particle index inside the replica, so replica r draws
+the numbers a single ``Simulation`` with that seed would draw.
+
+Rebuilds are collective: when any replica's skin trigger fires
+(neighbor_list.py:183-201), every replica rebuilds its list; each replica's
+update counter counts those rebuilds.
+"""
+from __future__ import annotations
+
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib
+from .engine import Engine
+from .potential import device_potential, lower_parameters
+
+
+class ReplicaBatch:
+    def __init__(self, simulations: Sequence, precision: str | None = None, device: int = 0,
+                 seeds: Sequence[int] | None = None, pair_precision: str | None = None):
+        sims = list(simulations)
+        if not sims:
+            raise ValueError("ReplicaBatch needs at least one Simulation")
+        self.simulations = sims
+        s0 = sims[0]
+        dim = s0.get_dimensions_of_space()
+        pot = (s0.pair_potential_str, float(s0.pair_potential_r_cut))
+        tabs = []
+        for s in sims:
+            if s.get_dimensions_of_space() != dim:
+                raise ValueError("all replicas must have the same dimension of space")
+            if (s.pair_potential_str, float(s.pair_potential_r_cut)) != pot:
+                raise ValueError("all replicas must use the same pair potential and r_cut")
+            types = np.asarray(s.particle_types).reshape(-1)
+            if types.shape[0] != s.number_of_particles():
+                types = np.zeros(s.number_of_particles(), dtype=np.int32)
+            tabs.append(lower_parameters(s.sigma_func, s.epsilon_func, types,
+                                         s.__dict__.get("_sigma_const"), s.__dict__.get("_epsilon_const")))
+        for t in tabs[1:]:
+            if t.sigma.shape != tabs[0].sigma.shape or not (np.array_equal(t.sigma, tabs[0].sigma)
+                                                            and np.array_equal(t.epsilon, tabs[0].epsilon)):
+                raise ValueError("all replicas must have the same sigma / epsilon type tables")
+        self.dim = dim
+        self.counts = np.array([s.number_of_particles() for s in sims], dtype=np.int64)
+        self.offsets = np.concatenate([[0], np.cumsum(self.counts)]).astype(np.int64)
+        n = int(self.offsets[-1])
+        R = len(sims)
+        prec = precision if precision is not None else s0.__dict__.get("_precision", "fp64")
+        pp = pair_precision if pair_precision is not None else s0.__dict__.get("_pair_precision", "fp64")
+        eng = Engine(dim, n, R, prec, device)
+        self.engine = eng
+        boxes = np.stack([np.asarray(s.box_vectors, dtype=np.float64).reshape(dim) for s in sims])
+        eng.set_replicas(self.offsets, boxes)
+        dp = device_potential(*pot)
+        eng.load_potential(dp.code, dp.r_cut, dp.u_rc, tabs[0].sigma, tabs[0].epsilon, pp)
+        # Simulation.py:353-356 per replica
+        cut = np.array([t.sigma_max * float(s.pair_potential_r_cut) + float(s.neighbor_list_skin)
+                        for s, t in zip(sims, tabs)])
+        skin = np.array([float(s.neighbor_list_skin) for s in sims])
+        K = int(max(int(s.max_number_of_neighbors) for s in sims))
+        order = _lib.DP_ORDER_X_FASTEST if dim == 3 else _lib.DP_ORDER_LAST_FASTEST
+        eng.set_neighbor_params(cut, skin, K, order)
+        eng.set_integrator([float(s.time_step) for s in sims], [float(s.temperature_target) for s in sims],
+                           [float(s.temperature_damping_time) for s in sims])
+        seeds = [int(s.seed) for s in sims] if seeds is None else [int(x) for x in seeds]
+        if len(seeds) != R:
+            raise ValueError("one seed per replica")
+        eng.set_seed(seeds[0], int(s0.number_of_steps))
+        eng.set_replica_seeds(seeds)
+        self.seeds = seeds
+
+        def cat(name, width, dtype):
+            parts = []
+            for s in sims:
+                a = np.asarray(getattr(s, name), dtype=dtype)
+                want = (s.number_of_particles(), width) if width else (s.number_of_particles(),)
+                if a.size != int(np.prod(want)):
+                    raise ValueError(f"{name} of a replica has shape {a.shape}, expected {want}")
+                parts.append(a.reshape(want))
+            return np.ascontiguousarray(np.concatenate(parts))
+
+        eng.set_field("positions", cat("positions", dim, np.float64))
+        eng.set_field("velocities", cat("velocities", dim, np.float64))
+        eng.set_field("betas", cat("betas", dim, np.float64))
+        eng.set_field("image_positions", cat("image_positions", dim, np.int32))
+        eng.set_field("masses", cat("masses", 0, np.float64))
+        eng.set_field("types", np.ascontiguousarray(np.concatenate([t.types for t in tabs])))
+        eng.build_neighbor_list()
+        self.number_of_steps = int(s0.number_of_steps)
+        # the build above is this context's own start, not a reference rebuild
+        self._updates0 = np.array([s.number_of_neighbor_list_updates for s in sims], dtype=np.int64)
+
+    # -- stepping -----------------------------------------------------------
+    @property
+    def n_replicas(self) -> int:
+        return len(self.simulations)
+
+    def run(self, steps: int = 1000) -> None:
+        """``steps`` G-JF steps of every replica (Simulation.py:726-737)."""
+        if steps > 0:
+            self.engine.step(int(steps), "gjf")
+            self.number_of_steps += int(steps)
+
+    def step(self, noise=None) -> None:
+        """One G-JF step of every replica; ``noise`` [N_total, D] optional."""
+        self.engine.step(1, "gjf", xi=noise)
+        self.number_of_steps += 1
+
+    def step_leap_frog(self) -> None:
+        self.engine.step(1, "leapfrog")
+        self.number_of_steps += 1
+
+    # -- observables ----------------------------------------------------------
+    def _split(self, a):
+        return [a[self.offsets[r]:self.offsets[r + 1]] for r in range(self.n_replicas)]
+
+    def get_potential_energy(self) -> np.ndarray:
+        """Per-replica potential energies (potential.py:69-100)."""
+        return np.array(self.engine.compute(forces=False, energy=True)["energy"])
+
+    def get_virial(self) -> np.ndarray:
+        """Per-replica virials (potential.py:253-279 over the list)."""
+        return np.array(self.engine.compute(forces=False, virial=True)["virial"])
+
+    def get_forces(self) -> list:
+        """Per-replica force arrays (potential.py:163-188)."""
+        return self._split(self.engine.compute(forces=True)["forces"])
+
+    def get_field(self, name: str) -> list:
+        """Per-replica copies of a state array ("positions", "velocities", ...)."""
+        return self._split(self.engine.get_field(name))
+
+    def number_of_neighbor_list_updates(self) -> np.ndarray:
+        upd, _, _ = self.engine.stats()
+        return self._updates0 + upd
+
+    def write_back(self) -> None:
+        """Copy every replica's state back into its Simulation object."""
+        fields = {f: self.get_field(f) for f in ("positions", "velocities", "betas", "image_positions")}
+        upd = self.number_of_neighbor_list_updates()
+        for r, s in enumerate(self.simulations):
+            for f, parts in fields.items():
+                setattr(s, f, np.array(parts[r]))
+            s.__dict__["number_of_steps"] = self.number_of_steps
+            s.number_of_neighbor_list_updates = int(upd[r])
+
+    def close(self) -> None:
+        self.engine.close()

@@ -50,6 +50,7 @@ struct DpRep {
   double skin_half_sq;  // (0.5 * skin)^2
   double dt, T_, tau;   // integrator parameters
   double g1_sq, g1_ca, g1_cb;  // G-JF coefficients for mass 1 (integrator.py:17-40 order, on the host)
+  u64 seed;             // noise key of the replica (dp_set_seed / dp_set_replica_seeds)
   i32 nhalo;            // prod(Th)
   i32 nhome;            // prod(T)
   i32 collapsed_mask;   // bit d set if nc[d] == 1

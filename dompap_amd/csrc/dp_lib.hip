@@ -221,6 +221,7 @@ struct dp_ctx {
   std::vector<i64> cells_sig, tiles_sig;  // geometry the list was built for
   bool pos_dirty = true;
   u64 seed = 0x5eed0d0bull;
+  std::vector<u64> rep_seeds;  // per replica; empty: every replica uses `seed`
 
   // graphs
   int graph_block = 32;
@@ -433,6 +434,7 @@ int refresh_reps(dp_ctx* ctx) {
     P.dt = ctx->have_integ ? ctx->dt[r] : 0.01;
     P.T_ = ctx->have_integ ? ctx->T[r] : 1.0;
     P.tau = ctx->have_integ ? ctx->tau[r] : 0.1;
+    P.seed = ((int)ctx->rep_seeds.size() == ctx->R) ? ctx->rep_seeds[r] : ctx->seed;
     {
       // G-JF coefficients of a unit mass, in integrator.py:17-40's order
       // (IEEE operations, so the device's per-particle values are identical)
@@ -693,7 +695,7 @@ DpPairArgs pair_args(dp_ctx* ctx, int cur) {
   a.orig = ctx->orig;
   a.rep = ctx->rep;
   a.xi = nullptr;
-  a.seed = ctx->seed;
+  a.seed = ctx->seed;  // (the kernels key the noise with each replica's DpRep::seed)
   a.ctl = ctx->ctl_d;
   a.wrap = 1;
   return a;
@@ -1514,11 +1516,21 @@ DP_API int dp_set_seed(dp_ctx* ctx, uint64_t seed, uint64_t step_counter) {
   if (!ctx) return DP_ERR_ARG;
   (void)hipSetDevice(ctx->dev);
   ctx->seed = seed;
+  ctx->rep_seeds.clear();
+  ctx->reps_dirty = true;  // replica keys live in the replica table
   HIPCHK(hipStreamSynchronize(ctx->st));
   HIPCHK(hipMemcpy(ctx->ctl_h, ctx->ctl_d, sizeof(DpCtl), hipMemcpyDeviceToHost));
   ctx->ctl_h->step_counter = step_counter;
   HIPCHK(hipMemcpy(ctx->ctl_d, ctx->ctl_h, sizeof(DpCtl), hipMemcpyHostToDevice));
   release_graph(ctx);  // the seed is a kernel argument
+  return DP_OK;
+}
+
+DP_API int dp_set_replica_seeds(dp_ctx* ctx, const uint64_t* seeds) {
+  if (!ctx || !seeds) return DP_ERR_ARG;
+  ctx->rep_seeds.assign(seeds, seeds + ctx->R);
+  ctx->reps_dirty = true;
+  release_graph(ctx);
   return DP_OK;
 }
 

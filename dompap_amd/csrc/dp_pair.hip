@@ -482,7 +482,10 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
         for (int j = 0; j < DP_JMAX; ++j) {
           const int d = sub + j * DP_TPP;
           z_own[j] = 0.0;
-          if (d < DP_D) z_own[j] = a.xi ? a.xi[(i64)pid * DP_D + d] : dp_normal(a.seed, a.ctl->step_index, pid, d);
+          // device noise: keyed by the replica's seed and the particle's id in its replica
+          if (d < DP_D)
+            z_own[j] = a.xi ? a.xi[(i64)pid * DP_D + d]
+                            : dp_normal(P.seed, a.ctl->step_index, (u32)(pid - (u32)P.p0), d);
         }
       }
       double sq = 0.0, ca = 0.0, cb = 0.0;
@@ -633,7 +636,7 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
 #pragma unroll
         for (int k = 0; 2 * k < DP_D; ++k) {
           double z0, z1;
-          dp_normal2(a.seed, step, pid, k, z0, z1);
+          dp_normal2(P.seed, step, (u32)(pid - (u32)P.p0), k, z0, z1);  // replica seed, id in replica
           z[2 * k] = z0;
           if (2 * k + 1 < DP_D) z[2 * k + 1] = z1;
         }

@@ -124,6 +124,10 @@ class Engine:
     def set_seed(self, seed: int, step_counter: int = 0) -> None:
         self._chk(self.lib.dp_set_seed(self.ctx, int(seed) & (2**64 - 1), int(step_counter)))
 
+    def set_replica_seeds(self, seeds) -> None:
+        a = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64).reshape(self.n_replicas))
+        self._chk(self.lib.dp_set_replica_seeds(self.ctx, _ptr(a)))
+
     def set_tuning(self, threads_per_particle: int = 0, tile_particles: int = 0, fused: int = -1) -> None:
         """Kernel shape: lanes per particle, home particles per tile (0 =
         planner's choice for both) and fused (1) / split (0) step kernels

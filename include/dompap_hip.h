@@ -143,6 +143,11 @@ int dp_get_plan(dp_ctx* ctx, int32_t* out, int n);
 
 /* RNG key and step counter. */
 int dp_set_seed(dp_ctx* ctx, uint64_t seed, uint64_t step_counter);
+/* Noise key per replica (overrides dp_set_seed's key; dp_set_seed resets to
+ * one key).  No counterpart in the reference: with it, replica r of a batch
+ * draws the same numbers as a single-replica context keyed with seeds[r]
+ * (numbers are keyed by the particle's index inside its replica). */
+int dp_set_replica_seeds(dp_ctx* ctx, const uint64_t* seeds);
 
 /* Neighbour list in the reference layout int32 [N][K], -1 padded
  * (Simulation.neighbor_list); rows in stencil order, members in ascending id.
