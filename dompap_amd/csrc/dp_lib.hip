@@ -339,6 +339,9 @@ int refresh_reps(dp_ctx* ctx) {
   ctx->fused = ctx->tune_fused >= 0 ? ctx->tune_fused : 0;
   // home particles per tile: one pass of the workgroup's lane groups
   const int target = ctx->tune_tile > 0 ? ctx->tune_tile : DP_TILE_THREADS / tpp;
+  // LDS record capacity of the pair kernels (48 KB of records)
+  const size_t rec_bytes_plan = (ctx->mode == DP_MIXED) ? (size_t)ctx->rs * 4 : (size_t)D * 8 + 4;
+  const i64 lds_rec_max = (i64)(48 * 1024 / rec_bytes_plan);
   i64 c0 = 0, t0 = 0;
   double rmax = 1.0;   // largest |coordinate offset| or displacement, for the fixed-point scale
   double occ_max = 0.0, halo_occ_max = 0.0;
@@ -388,7 +391,11 @@ int refresh_reps(dp_ctx* ctx) {
       if (best < 0) break;
       const i64 nT = prodT / T[best] * (T[best] + 1);
       const i64 nH = prodH / (T[best] + 2) * (T[best] + 3);
-      if (occ * (double)nT > 1.1 * target || nT > DP_HOME_MAX || nH > DP_HALO_MAX) break;
+      // stop at a full pass of lane groups, the table limits, or a halo that
+      // would not fit the LDS record area (1.5x its mean occupancy + 64)
+      if (occ * (double)nT > 1.1 * target || nT > DP_HOME_MAX || nH > DP_HALO_MAX ||
+          1.5 * occ * (double)nH + 64.0 > (double)lds_rec_max)
+        break;
       T[best] += 1;
     }
     i64 ntiles = 1, nhome = 1, nhalo = 1;
