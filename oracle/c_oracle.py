@@ -97,9 +97,11 @@ def pair(x, box, nl, pot, r_cut, sigma=1.0, eps=1.0, types=None, forces=True, en
     box = np.ascontiguousarray(box, dtype=np.float64)
     nl = np.ascontiguousarray(nl, dtype=np.int32)
     N, D = x.shape
-    sig = np.ascontiguousarray(np.atleast_2d(np.asarray(sigma, dtype=np.float64)))
-    ep = np.ascontiguousarray(np.atleast_2d(np.asarray(eps, dtype=np.float64)))
-    T = sig.shape[0]
+    sig = np.atleast_2d(np.asarray(sigma, dtype=np.float64))
+    ep = np.atleast_2d(np.asarray(eps, dtype=np.float64))
+    T = max(sig.shape[0], ep.shape[0])  # a scalar table applies to every type pair
+    sig = np.ascontiguousarray(np.broadcast_to(sig, (T, T)))
+    ep = np.ascontiguousarray(np.broadcast_to(ep, (T, T)))
     ty = None if types is None else np.ascontiguousarray(types, dtype=np.int32).reshape(-1)
     F = np.empty((N, D)) if forces else None
     E = np.empty(N) if energy else None

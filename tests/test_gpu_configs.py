@@ -25,13 +25,17 @@ def _sorted_rows(nl):
 
 
 def _check_against_oracle(s, pot, r_cut, cutoff, sigma=1.0, types=None):
-    x = np.asarray(s.positions)
     box = np.asarray(s.box_vectors, dtype=np.float64)
     K = int(s.max_number_of_neighbors)
-    nlo, cnt = C.neighbor_list(x, box, cutoff, K, "cells")
+    # the list is the one built at the last rebuild (positions_neighbour_list)
     nlg = np.asarray(s.neighbor_list)
+    xref = np.asarray(s.positions_neighbour_list)
+    nlo, cnt = C.neighbor_list(xref, box, cutoff, K, "cells")
     assert np.array_equal(_sorted_rows(nlg), _sorted_rows(nlo))
-    o = C.pair(x, box, nlo, pot, r_cut, sigma=sigma, types=types, energy=True, virial=True)
+    # forces at the current positions (every pair within r_cut is listed)
+    x = np.asarray(s.positions)
+    nlx, _ = C.neighbor_list(x, box, cutoff, K, "cells")
+    o = C.pair(x, box, nlx, pot, r_cut, sigma=sigma, types=types, energy=True, virial=True)
     F = s.get_forces()
     assert np.abs(F - o["forces"]).max() / _frms(o["forces"]) < 1e-5
     E = s.get_potential_energy()
@@ -72,6 +76,7 @@ def test_config4a_4d_harmonic_2m():
     a = (1 / 1.2) ** (1 / 4)
     s.set_positions(unit_cell_coordinates=([0, 0, 0, 0],), cells=(32, 32, 32, 64), lattice_constants=(a,) * 4)
     assert s.number_of_particles() == 2 ** 21
+    s.set_masses(1.0)
     s.positions = np.asarray(s.positions) + rng.normal(0, 0.1 * a, size=s.positions.shape)
     s.velocities = rng.normal(0, np.sqrt(0.001), size=s.positions.shape)
     s.set_pair_potential("(1-r)**2", r_cut=1.0)
@@ -89,6 +94,7 @@ def test_config4b_2d_ipl_4m():
     s = Simulation(seed=55, precision="mixed")
     s.set_positions(unit_cell_coordinates=([0, 0],), cells=(2048, 2048), lattice_constants=(1.0, 1.0))
     assert s.number_of_particles() == 2 ** 22
+    s.set_masses(1.0)
     s.positions = np.asarray(s.positions) + rng.normal(0, 0.05, size=s.positions.shape)
     s.velocities = rng.normal(0, 1.0, size=s.positions.shape)
     s.set_pair_potential("r**-12", r_cut=1.5)

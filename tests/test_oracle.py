@@ -404,6 +404,30 @@ def test_c_oracle_matches_numpy_oracle(dim, pot, r_cut, shape):
     assert np.abs(out["laplacian"] - Lo).max() <= 1e-10 * max(1.0, np.abs(Lo).max())
 
 
+def test_c_oracle_typed_tables_match_numpy_oracle():
+    """Per-type sigma / epsilon tables (a Wahnstrom-type mixture, epsilon
+    given as a scalar and as a table) agree with the callables of the NumPy
+    oracle (potential.py:163-188 with sigma(n, m), epsilon(n, m))."""
+    rng = np.random.default_rng(11)
+    a = 1.1
+    x = R.generate_positions(np.zeros((1, 3)), (8, 8, 8), (a,) * 3) + rng.normal(0, 0.05, (512, 3))
+    box = np.full(3, 8 * a)
+    x = x % box
+    types = np.zeros(512, dtype=np.int32)
+    types[rng.permutation(512)[:256]] = 1
+    sig = np.array([[1.0, (1 + 1 / 1.2) / 2], [(1 + 1 / 1.2) / 2, 1 / 1.2]])
+    eps = np.array([[1.0, 1.5], [1.5, 0.5]])
+    nl = R.neighbor_list_double_loop(x, box, 3.0, 300)
+    u, f, c = R.make_pair_potential("4*(r**-12-r**-6)", 2.5)
+    sf = lambda i, j: sig[types[i], types[j]]  # noqa: E731
+    for e_arg, ef in ((1.0, ONE), (eps, lambda i, j: eps[types[i], types[j]])):
+        out = C.pair(x, box, nl, "lj", 2.5, sigma=sig, eps=e_arg, types=types, energy=True)
+        Fo = R.forces(x, box, f, nl, sf, ef)
+        assert np.abs(out["forces"] - Fo).max() / max(1.0, np.abs(Fo).max()) < 1e-12
+        Eo = R.total_energy(x, box, u, nl, sf, ef)
+        assert abs(out["energy"] - Eo) <= 1e-12 * abs(Eo)
+
+
 # ---------------------------------------------------------------- pinning works
 def test_planted_oracle_mistakes_are_caught(monkeypatch):
     """A plausible mistake planted in the oracle fails one of the checks."""
