@@ -164,6 +164,8 @@ class Simulation:
         d["_params_dirty"] = True
         d["_integ_dirty"] = True
         d["_pot_dirty"] = True
+        d["_tuning"] = (0, 0, -1)  # kernel shape: planner's choice
+        d["_tuning_dirty"] = True
         d["_tables"] = None
         d["_nl_pending"] = False
         d["_nl_host_count"] = 0
@@ -300,6 +302,7 @@ class Simulation:
                 eng.close()
             eng = Engine(dim, n, 1, self._precision, self._device_index)
             self.__dict__["_engine"] = eng
+            self.__dict__["_tuning_dirty"] = True
             self.__dict__["_dev_count_base"] = 0
             eng.set_seed(self.seed, self.number_of_steps)
             self._dirty.update(_DEVICE_FIELDS + _HOST_FIELDS)
@@ -323,6 +326,9 @@ class Simulation:
             eng.set_integrator(float(self.time_step), float(self.temperature_target),
                                float(self.temperature_damping_time))
             self.__dict__["_integ_dirty"] = False
+        if self.__dict__.get("_tuning_dirty", True):
+            eng.set_tuning(*self.__dict__.get("_tuning", (0, 0, -1)))
+            self.__dict__["_tuning_dirty"] = False
         bad = self.__dict__.setdefault("_bad_fields", {})
         for f in ("positions", "velocities", "betas", "image_positions", "masses"):
             if f in self._dirty:
@@ -441,6 +447,7 @@ class Simulation:
         nd["_params_dirty"] = True
         nd["_integ_dirty"] = True
         nd["_pot_dirty"] = True
+        nd["_tuning_dirty"] = True
         if self._engine is not None and not self._nl_pending:
             nd["_restore_xref"] = self._engine.get_field("positions_neighbour_list")
             nd["_nl_pending"] = False
@@ -661,6 +668,19 @@ class Simulation:
             self.temperature_target = np.float64(target_temperature)
         if temperature_damping_time is not None:
             self.temperature_damping_time = np.float64(temperature_damping_time)
+
+    def set_kernel_tuning(self, threads_per_particle: int = 0, tile_particles: int = 0, fused: int = -1) -> None:
+        """GPU kernel shape (no counterpart in the reference; set by
+        dompap_amd.tools.autotune): lanes per particle (0: planner), home
+        particles per tile (0: planner), fused force + integrate kernel (1),
+        split (0) or planner's choice (-1).  Results do not depend on it
+        beyond summation order."""
+        self.__dict__["_tuning"] = (int(threads_per_particle), int(tile_particles), int(fused))
+        self.__dict__["_tuning_dirty"] = True
+
+    def get_kernel_plan(self) -> dict:
+        """The tile plan in use (lanes per particle, entry width, tiles, ...)."""
+        return self._device().plan()
 
     def number_of_particles(self) -> int:
         return self._shape()[0]

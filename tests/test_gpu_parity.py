@@ -398,3 +398,21 @@ def test_million_random_parity_against_c_oracle():
     assert abs(res["mixed"]["energy"][0] - o["energy"]) / abs(o["energy"]) < 1e-6
     # Newton's third law at full size
     assert np.abs(res["fp64"]["forces"].sum(0)).max() / frms < 1e-8
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fused_and_split_step_kernels_agree(precision):
+    """The fused force + integrate kernel and the force-then-integrate pair
+    run the same arithmetic in the same lane order: identical trajectories."""
+    runs = []
+    for fused in (0, 1):
+        s, x = _lj_system(precision, cells=6, density=0.9)
+        s.set_masses(1.0)
+        s.set_integrator(time_step=0.005, target_temperature=1.0, temperature_damping_time=1.0)
+        s.set_kernel_tuning(2, 0, fused)
+        s.velocities = np.random.default_rng(9).normal(0, 1, size=x.shape)
+        s.betas = np.zeros_like(x)
+        s.run(30)
+        runs.append((np.asarray(s.positions).copy(), np.asarray(s.velocities).copy()))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
