@@ -457,7 +457,8 @@ __device__ void dp_tile_setup(DpTileSmem<D>& S, const DpRep* reps, int R, const 
 // Tile tables in global memory, written by the neighbour-list emit at every
 // build (cell contents are fixed between builds) and loaded by the per-step
 // kernels with one coalesced copy instead of recomputing them:
-//   [0] nrec [1] nhomep [2] rep [3] maxc [4] nhalo [5] nhome [6..7] unused,
+//   [0] nrec [1] nhomep [2] rep [3] maxc [4] nhalo [5] nhome
+//   [6] 1 if the halo exceeds the emit's LDS record area [7] unused,
 //   then hstart[nh], hbase[nh + 1], hcoord[nh], home_u[nhm], home_p[nhm + 1].
 #define DP_TT_HEAD 8
 __host__ __device__ __forceinline__ int dp_tile_table_ints(int nh, int nhm) {

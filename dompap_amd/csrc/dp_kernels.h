@@ -433,8 +433,11 @@ __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const 
   }
 }
 
+// Tiles whose halo fits the LDS record area; the others are marked in their
+// table (word 6) and emitted by k_emit_tile_far.  Keeping the global-memory
+// path out of this kernel keeps it at <= 128 VGPRs (4 wavefronts per SIMD).
 template <int D, typename E>
-__global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile(const double* xref, const i32* cellpack, double qinv,
+__global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* xref, const i32* cellpack, double qinv,
                                                                const i32* cstart, const DpRep* reps, int R,
                                                                i64 ntiles, int K, int cap, E* nl, i32* nlcnt,
                                                                i32* tiletab, int tstride, DpCtl* ctl, int force) {
@@ -454,6 +457,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile(const double* xre
     const DpRep& P = reps[S.rep];
     const bool fits = S.fits != 0;
     dp_tile_store<D>(S, P, tiletab + t * (i64)tstride);
+    if (tid == 0) tiletab[t * (i64)tstride + 6] = fits ? 0 : 1;
     if (fits) {
       for (int idx = tid; idx < S.nrec; idx += blockDim.x) {
         const int u = dp_halo_of<D>(S, P.nhalo, idx);
@@ -471,9 +475,6 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile(const double* xre
     if (fits) {
       if (cm) dp_emit_particles<D, E, true, true>(S, P, xref, cellpack, qinv, srq, K, nl, nlcnt, ctl, st_count);
       else dp_emit_particles<D, E, true, false>(S, P, xref, cellpack, qinv, srq, K, nl, nlcnt, ctl, st_count);
-    } else {
-      if (cm) dp_emit_particles<D, E, false, true>(S, P, xref, cellpack, qinv, srq, K, nl, nlcnt, ctl, st_count);
-      else dp_emit_particles<D, E, false, false>(S, P, xref, cellpack, qinv, srq, K, nl, nlcnt, ctl, st_count);
     }
     __syncthreads();
   }
@@ -488,6 +489,57 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile(const double* xre
     atomicMax(sl + DP_STAT_CELL, st_cell);
     atomicMax(sl + DP_STAT_HALO, st_halo);
   }
+}
+
+// Emit for the tiles marked by k_emit_tile (halo larger than the LDS record
+// area): the same emit reading records from global memory.
+template <int D, typename E>
+__global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile_far(const double* xref, const i32* cellpack,
+                                                                   double qinv, const DpRep* reps, i64 ntiles, int K,
+                                                                   E* nl, i32* nlcnt, const i32* tiletab, int tstride,
+                                                                   DpCtl* ctl, int force) {
+  if (!force && !ctl->rebuild) return;
+  if (ctl->nonfinite) return;
+  __shared__ DpTileSmem<D> S;
+  int st_count = 0;
+  for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const i32* tt = tiletab + t * (i64)tstride;
+    if (tt[6] == 0) continue;  // uniform over the workgroup
+    dp_tile_load<D>(S, tt, 0);
+    const DpRep& P = reps[S.rep];
+    // st_off is not part of the stored tables
+    constexpr int n3 = DpPow3<D>::value;
+    for (int st = threadIdx.x; st < n3; st += blockDim.x) {
+      int sd[D];
+      dp_stencil_digits<D>(st, P.order, sd);
+      int off = 0, stride = 1;
+      if (P.order == 0) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          if (P.nc[d] != 1) off += sd[d] * stride;
+          stride *= P.Th[d];
+        }
+      } else {
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d) {
+          if (P.nc[d] != 1) off += sd[d] * stride;
+          stride *= P.Th[d];
+        }
+      }
+      S.st_off[st] = off;
+    }
+    __syncthreads();
+    if (P.collapsed_mask != 0)
+      dp_emit_particles<D, E, false, true>(S, P, xref, cellpack, qinv, nullptr, K, nl, nlcnt, ctl, st_count);
+    else
+      dp_emit_particles<D, E, false, false>(S, P, xref, cellpack, qinv, nullptr, K, nl, nlcnt, ctl, st_count);
+    __syncthreads();
+  }
+  int mx = st_count;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
+  if ((threadIdx.x & 63) == 0 && mx > 0)
+    atomicMax(ctl->slot + (blockIdx.x % DP_STAT_SLOTS) * DP_STAT_STRIDE + DP_STAT_COUNT, mx);
 }
 
 __global__ void k_clear_stats(DpCtl* ctl) {

@@ -340,8 +340,10 @@ int refresh_reps(dp_ctx* ctx) {
   // home particles per tile: one pass of the workgroup's lane groups
   const int target = ctx->tune_tile > 0 ? ctx->tune_tile : DP_TILE_THREADS / tpp;
   // LDS record capacity of the pair kernels (48 KB of records)
-  const size_t rec_bytes_plan = (ctx->mode == DP_MIXED) ? (size_t)ctx->rs * 4 : (size_t)D * 8 + 4;
-  const i64 lds_rec_max = (i64)(48 * 1024 / rec_bytes_plan);
+  // staged record: mixed RS ints; FP64 D doubles (+ the type with several types)
+  const size_t rec_bytes = (ctx->mode == DP_MIXED) ? (size_t)ctx->rs * 4
+                                                   : (size_t)D * 8 + (ctx->ntypes > 1 ? 4 : 0);
+  const i64 lds_rec_max = (i64)(48 * 1024 / rec_bytes);
   i64 c0 = 0, t0 = 0;
   double rmax = 1.0;   // largest |coordinate offset| or displacement, for the fixed-point scale
   double occ_max = 0.0, halo_occ_max = 0.0;
@@ -391,10 +393,11 @@ int refresh_reps(dp_ctx* ctx) {
       if (best < 0) break;
       const i64 nT = prodT / T[best] * (T[best] + 1);
       const i64 nH = prodH / (T[best] + 2) * (T[best] + 3);
-      // stop at a full pass of lane groups, the table limits, or a halo that
-      // would not fit the LDS record area (1.5x its mean occupancy + 64)
+      // stop at a full pass of lane groups, the table limits, or a mean halo
+      // that would not fit the LDS record area (tiles above the mean by more
+      // than the area's slack take the global-memory paths)
       if (occ * (double)nT > 1.1 * target || nT > DP_HOME_MAX || nH > DP_HALO_MAX ||
-          1.5 * occ * (double)nH + 64.0 > (double)lds_rec_max)
+          occ * (double)nH + 32.0 > (double)lds_rec_max)
         break;
       T[best] += 1;
     }
@@ -495,7 +498,6 @@ int refresh_reps(dp_ctx* ctx) {
   if (halo_occ_max * 4.0 + 64.0 >= 65536.0) nlw = 32;
   const int ob = nlw;
   // LDS records: 1.5x the mean halo, at most 48 KB
-  const size_t rec_bytes = (ctx->mode == DP_MIXED) ? (size_t)ctx->rs * 4 : (size_t)D * 8 + 4;
   i64 cap = (i64)std::ceil(1.5 * halo_occ_max) + 64;
   cap = std::min<i64>(cap, (i64)(48 * 1024 / rec_bytes));
   cap = std::max<i64>(cap, 64);
@@ -726,18 +728,22 @@ int launch_flat(dp_ctx* ctx, hipFunction_t f, DpPairArgs& a) {
 
 // Neighbour-list emit over the current cells from xref; force = 1 runs it
 // outside a rebuild (same list, new tile plan).
-template <int D>
-int launch_emit_t(dp_ctx* ctx, int force) {
-  if (ctx->nlw == 16)
-    k_emit_tile<D, u16><<<ctx->tile_grid, DP_TILE_THREADS, ctx->lds_emit, ctx->st>>>(
-        ctx->xref, ctx->cellpack, std::ldexp(1.0, ctx->qbits), ctx->cstart, ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, ctx->cap_emit, (u16*)ctx->nl, ctx->nlcnt,
-        ctx->tiletab, ctx->tstride, ctx->ctl_d, force);
-  else
-    k_emit_tile<D, u32><<<ctx->tile_grid, DP_TILE_THREADS, ctx->lds_emit, ctx->st>>>(
-        ctx->xref, ctx->cellpack, std::ldexp(1.0, ctx->qbits), ctx->cstart, ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, ctx->cap_emit, (u32*)ctx->nl, ctx->nlcnt,
-        ctx->tiletab, ctx->tstride, ctx->ctl_d, force);
+template <int D, typename E>
+int launch_emit_te(dp_ctx* ctx, int force) {
+  const double qinv = std::ldexp(1.0, ctx->qbits);
+  k_emit_tile<D, E><<<ctx->tile_grid, DP_TILE_THREADS, ctx->lds_emit, ctx->st>>>(
+      ctx->xref, ctx->cellpack, qinv, ctx->cstart, ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, ctx->cap_emit,
+      (E*)ctx->nl, ctx->nlcnt, ctx->tiletab, ctx->tstride, ctx->ctl_d, force);
+  k_emit_tile_far<D, E><<<ctx->tile_grid, DP_TILE_THREADS, 0, ctx->st>>>(
+      ctx->xref, ctx->cellpack, qinv, ctx->reps_d, ctx->ntiles, ctx->K, (E*)ctx->nl, ctx->nlcnt, ctx->tiletab,
+      ctx->tstride, ctx->ctl_d, force);
   HIPCHK(hipGetLastError());
   return DP_OK;
+}
+
+template <int D>
+int launch_emit_t(dp_ctx* ctx, int force) {
+  return ctx->nlw == 16 ? launch_emit_te<D, u16>(ctx, force) : launch_emit_te<D, u32>(ctx, force);
 }
 
 int launch_reemit(dp_ctx* ctx) {
@@ -1275,6 +1281,7 @@ DP_API int dp_load_potential(dp_ctx* ctx, const char* device_src, int even, doub
   release_graph(ctx);
   DpPairTab& T = ctx->tab_h;
   std::memset(&T, 0, sizeof(T));
+  if (ntypes != ctx->ntypes) ctx->reps_dirty = true;  // FP64 records carry the type with several types
   ctx->ntypes = ntypes;
   for (int a = 0; a < ntypes; ++a)
     for (int b = 0; b < ntypes; ++b) {

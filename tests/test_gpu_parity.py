@@ -416,3 +416,38 @@ def test_fused_and_split_step_kernels_agree(precision):
         runs.append((np.asarray(s.positions).copy(), np.asarray(s.velocities).copy()))
     assert np.array_equal(runs[0][0], runs[1][0])
     assert np.array_equal(runs[0][1], runs[1][1])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_inhomogeneous_tiles_beyond_lds_capacity(precision):
+    """Half of the box empty: the planner sizes the LDS record area from the
+    mean occupancy, so the dense tiles take the global-memory paths of the
+    emit and the pair kernels; lists and forces must not change."""
+    from dompap_amd import Simulation
+    rng = np.random.default_rng(31)
+    g = np.arange(12)
+    half = np.stack(np.meshgrid(g, np.arange(24), np.arange(24), indexing="ij"), -1).reshape(-1, 3) * 0.95
+    x = half + rng.normal(0, 0.05, size=half.shape)
+    box = np.array([24 * 0.95] * 3)
+    x = x % box
+    s = Simulation(seed=3, precision=precision)
+    s.box_vectors = box
+    s.positions = x
+    s.velocities = np.zeros_like(x)
+    s.betas = np.zeros_like(x)
+    s.image_positions = np.zeros(x.shape, dtype=np.int32)
+    s.set_masses(1.0)
+    s.particle_types = np.zeros((x.shape[0], 1), dtype=np.int32)
+    s.set_pair_potential("4*(r**-12-r**-6)", r_cut=2.5)
+    s.set_neighbor_list(skin=0.5, max_number_of_neighbors=256)
+    F = s.get_forces()
+    plan = s.get_kernel_plan()
+    nl = np.asarray(s.neighbor_list)
+    nlo, _ = C.neighbor_list(x, box, 3.0, 256, "cells")
+    assert np.array_equal(nl, nlo)
+    Fo = C.pair(x, box, nlo, "lj", 2.5)["forces"]
+    tol = 1e-10 if precision == "fp64" else 1e-5
+    assert np.abs(F - Fo).max() / _frms(Fo) < tol
+    # a tile inside the occupied half holds about twice the mean occupancy
+    dense_halo = 2.0 * x.shape[0] / plan["cells"] * plan["halo_cells_per_tile"]
+    assert dense_halo > plan["lds_records"]
