@@ -91,55 +91,102 @@ template <int D>
 __global__ void k_bin(const double* x, const i32* rep, const DpRep* reps, i64 N, i32* cell_of, i32* rank,
                       i32* count, DpCtl* ctl) {
   if (!ctl->rebuild) return;
-  DP_FOR(i, N) {
-    const DpRep& R = reps[rep[i]];
-    int c[D];
-    bool finite = true;
+  const int lane = threadIdx.x & 63;
+  for (i64 base = (i64)blockIdx.x * blockDim.x; base < N; base += (i64)gridDim.x * blockDim.x) {
+    const i64 i = base + threadIdx.x;
+    const bool valid = i < N;  // valid lanes form a prefix of the wavefront
+    i32 cid = -1;
+    if (valid) {
+      const DpRep& R = reps[rep[i]];
+      int c[D];
+      bool finite = true;
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      double xd = x[i * D + d];
-      if (!(xd - xd == 0.0)) { finite = false; xd = 0.0; }
-      xd = xd - floor(xd / R.L[d]) * R.L[d];
-      int cd = (int)floor(xd * R.inv_w[d]);
-      if (cd < 0) cd = 0;
-      if (cd >= R.nc[d]) cd = R.nc[d] - 1;
-      c[d] = cd;
+      for (int d = 0; d < D; ++d) {
+        double xd = x[i * D + d];
+        if (!(xd - xd == 0.0)) { finite = false; xd = 0.0; }
+        xd = xd - floor(xd / R.L[d]) * R.L[d];
+        int cd = (int)floor(xd * R.inv_w[d]);
+        if (cd < 0) cd = 0;
+        if (cd >= R.nc[d]) cd = R.nc[d] - 1;
+        c[d] = cd;
+      }
+      if (!finite) ctl->nonfinite = 1;
+      cid = (i32)(R.c0 + dp_linear<D>(c, R.nc, R.order));
+      cell_of[i] = cid;
     }
-    if (!finite) ctl->nonfinite = 1;
-    const i32 cid = (i32)(R.c0 + dp_linear<D>(c, R.nc, R.order));
-    cell_of[i] = cid;
-    rank[i] = atomicAdd(&count[cid], 1);
+    // Particles are in cell order from the previous build, so neighbouring
+    // lanes mostly share a cell: one atomic per run of equal cells, the
+    // run's lanes take consecutive ranks after it.
+    const i32 prev = __shfl_up(cid, 1);
+    const bool head = valid && (lane == 0 || prev != cid);
+    const u64 heads = __ballot(head);
+    const u64 vmask = __ballot(valid);
+    const u64 le = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    const u64 hle = heads & le;
+    const int start = hle ? 63 - __clzll(hle) : 0;
+    int b = 0;
+    if (head) {
+      const u64 above = heads & ~le;
+      const int next = above ? __ffsll((unsigned long long)above) - 1 : 64 - __clzll(vmask);
+      b = atomicAdd(&count[cid], next - lane);
+    }
+    b = __shfl(b, start);
+    if (valid) rank[i] = b + (lane - start);
   }
 }
 
-// Single-workgroup exclusive scan of the cell counts (DP_SCAN_THREADS
-// threads, each over a contiguous run of cells): cstart[c] = sum of the
-// counts before c, cstart[C] = N.  The counts are cleared for the next build
-// as they are read.
+// Single-workgroup exclusive scan of the cell counts: blocks of 4096 cells,
+// 4 per thread, wavefront scans combined through LDS, a running carry.
+// cstart[c] = sum of the counts before c, cstart[C] = N.  The counts are
+// cleared for the next build as they are read.
 #define DP_SCAN_THREADS 1024
 __global__ void __launch_bounds__(DP_SCAN_THREADS) k_scan_single(i32* count, i64 C, i32* cstart, i64 N,
                                                                   const DpCtl* ctl) {
   if (!ctl->rebuild) return;
-  __shared__ i32 sh[DP_SCAN_THREADS];
+  __shared__ i32 wsum[DP_SCAN_THREADS / 64];
+  __shared__ i32 btotal;
   const int t = threadIdx.x;
-  const i64 per = (C + DP_SCAN_THREADS - 1) / DP_SCAN_THREADS;
-  const i64 lo = min((i64)t * per, C), hi = min(lo + per, C);
-  i32 s = 0;
-  for (i64 c = lo; c < hi; ++c) s += count[c];
-  sh[t] = s;
-  __syncthreads();
-  for (int off = 1; off < DP_SCAN_THREADS; off <<= 1) {
-    const i32 add = (t >= off) ? sh[t - off] : 0;
+  const int lane = t & 63;
+  const int w = t >> 6;
+  constexpr int NW = DP_SCAN_THREADS / 64;
+  i32 carry = 0;
+  for (i64 b0 = 0; b0 < C; b0 += 4 * DP_SCAN_THREADS) {
+    const i64 c0 = b0 + 4 * (i64)t;
+    i32 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = (c0 + k < C) ? count[c0 + k] : 0;
+    const i32 sthr = v[0] + v[1] + v[2] + v[3];
+    i32 incl = sthr;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const i32 y = __shfl_up(incl, off);
+      if (lane >= off) incl += y;
+    }
+    if (lane == 63) wsum[w] = incl;
     __syncthreads();
-    sh[t] += add;
+    if (w == 0) {
+      const i32 xw = (lane < NW) ? wsum[lane] : 0;
+      i32 xi = xw;
+#pragma unroll
+      for (int off = 1; off < NW; off <<= 1) {
+        const i32 y = __shfl_up(xi, off);
+        if (lane >= off) xi += y;
+      }
+      if (lane < NW) wsum[lane] = xi - xw;  // exclusive prefix of the wavefront totals
+      if (lane == NW - 1) btotal = xi;
+    }
     __syncthreads();
-  }
-  i32 ex = sh[t] - s;
-  for (i64 c = lo; c < hi; ++c) {
-    const i32 v = count[c];
-    cstart[c] = ex;
-    ex += v;
-    count[c] = 0;
+    i32 ex = carry + wsum[w] + (incl - sthr);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (c0 + k < C) {
+        cstart[c0 + k] = ex;
+        ex += v[k];
+        count[c0 + k] = 0;
+      }
+    }
+    carry += btotal;
+    __syncthreads();  // wsum / btotal are rewritten by the next block
   }
   if (t == 0) cstart[C] = (i32)N;
 }
