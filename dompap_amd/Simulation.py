@@ -713,11 +713,23 @@ class Simulation:
         return float(numerator / denominator)
 
     def get_radial_distribution_function(self, r_bins: np.ndarray):
-        """Simulation.py:858-881."""
-        from .positions import get_radial_distribution_function
+        """Simulation.py:858-881: g(r) of all ordered pairs, self pairs
+        included, over the minimum-image distances.  The pair counts are
+        histogrammed on the GPU (dp_rdf_counts: the cell tiles of the current
+        list when the last edge is within reach, else all pairs); the
+        normalisation is positions.py:79-87's."""
+        from .positions import normalise_radial_histogram
 
-        p = np.asarray(self.positions)
-        return get_radial_distribution_function(p, p, np.asarray(self.box_vectors), r_bins)
+        r_bins = np.asarray(r_bins, dtype=np.float64)
+        if r_bins.ndim != 1 or r_bins.size < 2:
+            raise ValueError("r_bins must be a 1-D array of at least two bin edges")
+        if not np.all(np.diff(r_bins) >= 0):
+            raise ValueError("r_bins array must be sorted in ascending order.")
+        eng = self._device()
+        counts = eng.rdf_counts(r_bins).astype(np.int64)
+        n = self.number_of_particles()
+        return normalise_radial_histogram(counts, r_bins, n * n, np.asarray(self.box_vectors),
+                                          self.get_dimensions_of_space())
 
     def get_number_of_particles(self) -> int:
         return self._shape()[0]

@@ -643,6 +643,209 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_nl_export(const E* nl, cons
   }
 }
 
+// ---------------------------------------------------------------------------
+// Radial distribution function counts (positions.py:58-87): every ordered
+// pair (n, m) of one replica, n == m included, at the distance NumPy
+// computes: d = x_n - x_m; d -= L * round(d / L) (half to even); r =
+// sqrt(d0^2 + d1^2 + ...), products and sums rounded one at a time, summed
+// in dimension order.  np.histogram with explicit edges puts r in bin i when
+// edges[i] <= r < edges[i+1], the last bin closed on the right; NaN nowhere.
+// The kernels bin the squared distance s instead of r: with sqrt correctly
+// rounded and monotone, edges[i] <= sqrt(s) <=> s >= thr[i], where thr[i] is
+// the smallest double with sqrt(thr[i]) >= edges[i], and sqrt(s) <= edges[-1]
+// <=> s <= u_last, the largest double with sqrt(u_last) <= edges[-1] (both
+// found on the host).  No device sqrt, no dependence on its rounding.
+// ---------------------------------------------------------------------------
+#define DP_RDF_MAXBINS 4096
+
+// Bin of s among the thresholds of one chunk of edges (`closed`: the chunk
+// holds the last bin); -1 outside.  Equal edges leave empty bins, as in NumPy.
+__device__ __forceinline__ int dp_rdf_bin(double s, const double* thr, int ne, int closed, double u_last) {
+  if (!(s >= thr[0])) return -1;  // below, or NaN
+  if (!(s < thr[ne - 1]) && !(closed && s <= u_last)) return -1;
+  int lo = 0, hi = ne - 2;  // largest i <= ne - 2 with thr[i] <= s
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (thr[mid] <= s) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// d - L * rint(d / L).  The quotient is needed only near |d / L| = 1/2 and
+// beyond: below 0.5 - 2^-40 (checked with the product by 1 / L, within a few
+// ulps of the quotient) it rounds to zero, and d - L * 0 = d.
+__device__ __forceinline__ double dp_rdf_wrap(double t, double L, double inv_L) {
+  if (fabs(t * inv_L) < 0.5 - 0x1p-40) return t;
+  return dp_sub(t, dp_mul(L, rint(t / L)));
+}
+
+template <int D>
+__device__ __forceinline__ double dp_rdf_sq(const double* a, const double* b, const DpRep& P, const double* inv_L) {
+  double s = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double t = dp_rdf_wrap(dp_sub(a[d], b[d]), P.L[d], inv_L[d]);
+    s = (d == 0) ? dp_mul(t, t) : dp_add(s, dp_mul(t, t));
+  }
+  return s;
+}
+
+// Add the workgroup's LDS counts to the global histogram and clear them.
+__device__ __forceinline__ void dp_rdf_flush(u32* hl, int nb, unsigned long long* hist) {
+  __syncthreads();
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    if (hl[b]) {
+      atomicAdd(hist + b, (unsigned long long)hl[b]);
+      hl[b] = 0;
+    }
+  __syncthreads();
+}
+
+// The 32-bit LDS counts are flushed before the pairs a workgroup has binned
+// since its last flush could exceed 2^31.
+#define DP_RDF_FLUSH_PAIRS (1ull << 31)
+
+// All pairs, 256 x 256 blocks staged in LDS (any box, any size).
+template <int D>
+__global__ void __launch_bounds__(256) k_rdf_all(const double* x, i64 N, const DpRep* reps, const double* thr_g,
+                                                 int ne, int closed, double u_last, unsigned long long* hist) {
+  __shared__ double sx[256 * D];
+  __shared__ double thr[DP_RDF_MAXBINS + 1];
+  __shared__ u32 hl[DP_RDF_MAXBINS];
+  const DpRep& P = reps[0];
+  double inv_L[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) inv_L[d] = 1.0 / P.L[d];
+  const int nb = ne - 1;
+  for (int b = threadIdx.x; b < ne; b += blockDim.x) thr[b] = thr_g[b];
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) hl[b] = 0;
+  const i64 nblk = (N + 255) / 256;
+  unsigned long long pending = 0;
+  for (i64 pb = blockIdx.x; pb < nblk * nblk; pb += gridDim.x) {
+    const i64 ib = pb / nblk, jb = pb % nblk;
+    if (pending + 65536ull > DP_RDF_FLUSH_PAIRS) {
+      dp_rdf_flush(hl, nb, hist);
+      pending = 0;
+    }
+    pending += 65536ull;
+    __syncthreads();
+    const i64 j = jb * 256 + threadIdx.x;
+#pragma unroll
+    for (int d = 0; d < D; ++d) sx[threadIdx.x * D + d] = (j < N) ? x[j * D + d] : 0.0;
+    __syncthreads();
+    const i64 i = ib * 256 + threadIdx.x;
+    if (i < N) {
+      double xi[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) xi[d] = x[i * D + d];
+      const int nj = (int)min((i64)256, N - jb * 256);
+      for (int jj = 0; jj < nj; ++jj) {
+        const int bin = dp_rdf_bin(dp_rdf_sq<D>(xi, sx + jj * D, P, inv_L), thr, ne, closed, u_last);
+        if (bin >= 0) atomicAdd(&hl[bin], 1u);
+      }
+    }
+  }
+  dp_rdf_flush(hl, nb, hist);
+}
+
+// Largest squared displacement since the last rebuild (x - xref, minimum
+// image as in the skin check), one value per workgroup; NaN propagates.
+template <int D>
+__global__ void __launch_bounds__(DP_BLOCK) k_rdf_disp(const double* x, const double* xref, const DpRep* reps, i64 N,
+                                                       double* block_max) {
+  __shared__ double sh[DP_BLOCK / 64];
+  const DpRep& R = reps[0];
+  double m = 0.0;
+  DP_FOR(i, N) {
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      double t = dp_sub(x[i * D + d], xref[i * D + d]);
+      t = dp_minimage_gt(t, R.L[d], R.halfL[d]);
+      s = (d == 0) ? dp_mul(t, t) : dp_add(s, dp_mul(t, t));
+    }
+    if (!(s <= m)) m = s;
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double o = __shfl_xor(m, off);
+    if (!(o <= m)) m = o;
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < DP_BLOCK / 64; ++w)
+      if (!(sh[w] <= m)) m = sh[w];
+    block_max[blockIdx.x] = m;
+  }
+}
+
+// Cell tiles of the last rebuild: valid when every pair closer than the last
+// edge lies in neighbouring cells (last edge + twice the largest displacement
+// since the rebuild below the cell width, at least 3 cells per dimension).
+// One thread per home particle, the 3^D stencil cells of its halo.
+template <int D>
+__global__ void __launch_bounds__(DP_TILE_THREADS) k_rdf_cells(const double* x, const DpRep* reps, i64 ntiles,
+                                                               const i32* tiletab, int tstride,
+                                                               const double* thr_g, int ne,
+                                                               int closed, double u_last, unsigned long long* hist) {
+  __shared__ DpTileSmem<D> S;
+  __shared__ double thr[DP_RDF_MAXBINS + 1];
+  __shared__ u32 hl[DP_RDF_MAXBINS];
+  const int nb = ne - 1;
+  for (int b = threadIdx.x; b < ne; b += blockDim.x) thr[b] = thr_g[b];
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) hl[b] = 0;
+  constexpr int n3 = DpPow3<D>::value;
+  unsigned long long pending = 0;
+  for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    dp_tile_load<D>(S, tiletab + t * (i64)tstride, 0);
+    const DpRep& P = reps[S.rep];
+    // a home particle meets at most every staged record of the halo
+    const unsigned long long tile_pairs = (unsigned long long)S.nhomep * (unsigned long long)S.nrec;
+    if (pending + tile_pairs > DP_RDF_FLUSH_PAIRS) {
+      dp_rdf_flush(hl, nb, hist);
+      pending = 0;
+    }
+    pending += tile_pairs;
+    for (int st = threadIdx.x; st < n3; st += blockDim.x) {
+      int sd[D];
+      dp_stencil_digits<D>(st, P.order, sd);
+      int off = 0, stride = 1;
+      if (P.order == 0) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) { off += sd[d] * stride; stride *= P.Th[d]; }
+      } else {
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d) { off += sd[d] * stride; stride *= P.Th[d]; }
+      }
+      S.st_off[st] = off;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < S.nhomep; k += blockDim.x) {
+      const int m = dp_home_of<D>(S, P.nhome, k);
+      const int uh = S.home_u[m];
+      const i64 slot_i = S.hstart[uh] + (k - S.home_p[m]);
+      double xi[D], inv_L[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        xi[d] = x[slot_i * D + d];
+        inv_L[d] = 1.0 / P.L[d];
+      }
+      for (int st = 0; st < n3; ++st) {
+        const int u = uh + S.st_off[st];
+        const i64 s0 = S.hstart[u];
+        const int cnt = S.hbase[u + 1] - S.hbase[u];
+        for (int q = 0; q < cnt; ++q) {
+          const int bin = dp_rdf_bin(dp_rdf_sq<D>(xi, x + (s0 + q) * D, P, inv_L), thr, ne, closed, u_last);
+          if (bin >= 0) atomicAdd(&hl[bin], 1u);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  dp_rdf_flush(hl, nb, hist);
+}
+
 // Deterministic per-replica sums (fixed assignment of elements to threads and
 // a fixed-shape tree), one workgroup per replica.
 __global__ void k_reduce_rep(const double* vals, const DpRep* reps, double* out) {

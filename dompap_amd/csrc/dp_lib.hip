@@ -218,6 +218,7 @@ struct dp_ctx {
   int tstride = 0;
   size_t tiletab_bytes = 0;
   bool nl_valid = true;                 // false when the list buffer was reallocated
+  int rdf_path = -1;                    // path of the last dp_rdf_counts: 1 cell tiles, 0 all pairs
   std::vector<i64> cells_sig, tiles_sig;  // geometry the list was built for
   bool pos_dirty = true;
   u64 seed = 0x5eed0d0bull;
@@ -1595,6 +1596,125 @@ DP_API int dp_get_neighbor_counts(dp_ctx* ctx, int32_t* out) {
   k_to_orig<i32><<<grid_for(ctx->N), DP_BLOCK, 0, ctx->st>>>(ctx->nlcnt, (i32*)ctx->stage, ctx->orig, ctx->N, 1);
   HIPCHK(hipMemcpyAsync(out, ctx->stage, (size_t)ctx->N * 4, hipMemcpyDefault, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
+  return DP_OK;
+}
+
+namespace {
+template <int D>
+int rdf_max_disp(dp_ctx* ctx, double* disp_sq) {
+  const int g = grid_for(ctx->N);
+  double* bm = nullptr;
+  if (hipMalloc(&bm, (size_t)g * 8) != hipSuccess) return ctx->fail(DP_ERR_HIP, "cannot allocate %d doubles", g);
+  k_rdf_disp<D><<<g, DP_BLOCK, 0, ctx->st>>>(ctx->x[ctx->cur], ctx->xref, ctx->reps_d, ctx->N, bm);
+  std::vector<double> h((size_t)g);
+  int rc = DP_OK;
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(h.data(), bm, (size_t)g * 8, hipMemcpyDeviceToHost, ctx->st) != hipSuccess ||
+      hipStreamSynchronize(ctx->st) != hipSuccess)
+    rc = ctx->fail(DP_ERR_HIP, "displacement reduction failed");
+  (void)hipFree(bm);
+  double m = 0.0;
+  for (double v : h)
+    if (!(v <= m)) m = v;
+  *disp_sq = m;
+  return rc;
+}
+
+// Smallest double s with sqrt(s) >= e (correctly rounded, monotone sqrt):
+// the squared-distance threshold of bin edge e.
+double rdf_lower(double e) {
+  if (!(e > 0.0)) return -INFINITY;  // every s >= 0 qualifies
+  double s = e * e;
+  while (s > 0.0 && std::sqrt(s) >= e) s = std::nextafter(s, -INFINITY);
+  while (!(std::sqrt(s) >= e)) s = std::nextafter(s, INFINITY);
+  return s;
+}
+
+// Largest double s with sqrt(s) <= e.
+double rdf_upper(double e) {
+  if (e < 0.0) return -INFINITY;  // no s >= 0 qualifies
+  if (e == 0.0) return 0.0;
+  double s = e * e;
+  while (s < INFINITY && std::sqrt(s) <= e) s = std::nextafter(s, INFINITY);
+  while (!(std::sqrt(s) <= e)) s = std::nextafter(s, -INFINITY);
+  return s;
+}
+
+template <int D>
+int launch_rdf_t(dp_ctx* ctx, const double* edges, int ne, unsigned long long* hist_d, const double* thr_d,
+                 double u_last) {
+  // the tiles of the last rebuild hold every pair up to the last edge when
+  // edges[-1] + 2 * max displacement stays below every cell width
+  const DpRep& P = ctx->reps_h[0];
+  bool cells = ctx->nl_built;
+  for (int d = 0; d < D; ++d) cells = cells && P.nc[d] >= 3;
+  if (cells) {
+    double dsq = 0.0;
+    CHK(rdf_max_disp<D>(ctx, &dsq));
+    const double reach = edges[ne - 1] + 2.0 * std::sqrt(dsq);
+    for (int d = 0; d < D; ++d) cells = cells && reach <= P.w[d] * (1.0 - 1e-9);
+  }
+  // bins in chunks that fit the workgroup's LDS histogram
+  const int nb = ne - 1;
+  for (int c0 = 0; c0 < nb; c0 += DP_RDF_MAXBINS) {
+    const int nbc = std::min(DP_RDF_MAXBINS, nb - c0);
+    const int closed = (c0 + nbc == nb) ? 1 : 0;
+    if (cells)
+      k_rdf_cells<D><<<ctx->tile_grid, DP_TILE_THREADS, 0, ctx->st>>>(ctx->x[ctx->cur], ctx->reps_d, ctx->ntiles,
+                                                                      ctx->tiletab, ctx->tstride, thr_d + c0, nbc + 1,
+                                                                      closed, u_last, hist_d + c0);
+    else
+      k_rdf_all<D><<<2048, 256, 0, ctx->st>>>(ctx->x[ctx->cur], ctx->N, ctx->reps_d, thr_d + c0, nbc + 1, closed,
+                                              u_last, hist_d + c0);
+    HIPCHK(hipGetLastError());
+  }
+  ctx->rdf_path = cells ? 1 : 0;
+  return DP_OK;
+}
+}  // namespace
+
+DP_API int dp_rdf_counts(dp_ctx* ctx, const double* edges, int n_edges, uint64_t* counts) {
+  if (!ctx || !edges || !counts) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  if (ctx->R != 1) return ctx->fail(DP_ERR_ARG, "dp_rdf_counts works on single-replica contexts");
+  if (n_edges < 2) return ctx->fail(DP_ERR_ARG, "need at least 2 bin edges");
+  for (int k = 1; k < n_edges; ++k)
+    if (!(edges[k] >= edges[k - 1])) return ctx->fail(DP_ERR_ARG, "r_bins array must be sorted in ascending order.");
+  CHK(refresh_reps(ctx));  // cells and tiles only: no potential needed
+  std::vector<double> thr((size_t)n_edges);
+  for (int k = 0; k < n_edges; ++k) thr[k] = rdf_lower(edges[k]);
+  const double u_last = rdf_upper(edges[n_edges - 1]);
+  double* edges_d = nullptr;
+  unsigned long long* hist_d = nullptr;
+  if (hipMalloc(&edges_d, (size_t)n_edges * 8) != hipSuccess ||
+      hipMalloc(&hist_d, (size_t)(n_edges - 1) * 8) != hipSuccess) {
+    if (edges_d) (void)hipFree(edges_d);
+    return ctx->fail(DP_ERR_HIP, "cannot allocate the histogram");
+  }
+  int rc = DP_OK;
+  if (hipMemcpyAsync(edges_d, thr.data(), (size_t)n_edges * 8, hipMemcpyHostToDevice, ctx->st) != hipSuccess ||
+      hipMemsetAsync(hist_d, 0, (size_t)(n_edges - 1) * 8, ctx->st) != hipSuccess)
+    rc = ctx->fail(DP_ERR_HIP, "histogram set-up failed");
+  if (rc == DP_OK) {
+    switch (ctx->D) {
+      case 1: rc = launch_rdf_t<1>(ctx, edges, n_edges, hist_d, edges_d, u_last); break;
+      case 2: rc = launch_rdf_t<2>(ctx, edges, n_edges, hist_d, edges_d, u_last); break;
+      case 3: rc = launch_rdf_t<3>(ctx, edges, n_edges, hist_d, edges_d, u_last); break;
+      case 4: rc = launch_rdf_t<4>(ctx, edges, n_edges, hist_d, edges_d, u_last); break;
+    }
+  }
+  if (rc == DP_OK && (hipMemcpyAsync(counts, hist_d, (size_t)(n_edges - 1) * 8, hipMemcpyDeviceToHost, ctx->st) !=
+                          hipSuccess ||
+                      hipStreamSynchronize(ctx->st) != hipSuccess))
+    rc = ctx->fail(DP_ERR_HIP, "histogram read-back failed");
+  (void)hipFree(edges_d);
+  (void)hipFree(hist_d);
+  return rc;
+}
+
+DP_API int dp_get_rdf_path(dp_ctx* ctx, int* cells) {
+  if (!ctx || !cells) return DP_ERR_ARG;
+  *cells = ctx->rdf_path;
   return DP_OK;
 }
 

@@ -237,6 +237,19 @@ class Engine:
         self._chk(self.lib.dp_get_kernel_time(self.ctx, ctypes.byref(ms), ctypes.byref(n)))
         return float(ms.value), int(n.value)
 
+    def rdf_counts(self, edges) -> np.ndarray:
+        """Pair-distance histogram of the (single) replica over bin edges."""
+        e = np.ascontiguousarray(np.asarray(edges, dtype=np.float64))
+        out = np.zeros(max(e.size - 1, 0), dtype=np.uint64)
+        self._chk(self.lib.dp_rdf_counts(self.ctx, _ptr(e), int(e.size), _ptr(out)))
+        return out
+
+    def rdf_path(self) -> int:
+        """1 if the last rdf_counts ran on the cell tiles, 0 for all pairs."""
+        v = ctypes.c_int32(-1)
+        self._chk(self.lib.dp_get_rdf_path(self.ctx, ctypes.byref(v)))
+        return int(v.value)
+
     def kernel_breakdown(self):
         """(force-kernel ms, integrate-kernel ms) summed over the timed steps."""
         f = ctypes.c_double()

@@ -62,11 +62,17 @@ def get_radial_distribution_function(positions_n, positions_m, box_vectors, r_bi
         diff[:, :, i] -= box_length * np.round(diff[:, :, i] / box_length)
     distances = np.linalg.norm(diff, axis=2)
     histogram = np.histogram(distances, bins=r_bins)[0]
-    number_of_pairs = positions_n.shape[0] * positions_m.shape[0]
+    return normalise_radial_histogram(histogram, r_bins, positions_n.shape[0] * positions_m.shape[0],
+                                      box_vectors, positions_n.shape[1])
+
+
+def normalise_radial_histogram(histogram, r_bins, number_of_pairs, box_vectors, dimensions):
+    """positions.py:79-87: bin centres and g(r) = counts / shell volume /
+    number of pairs * box volume, the shell volume being the n-sphere
+    surface 2 pi^(D/2) / Gamma(D/2) r_mid^(D-1) times the bin width."""
     box_volume = np.prod(box_vectors)
     dr_bins = np.diff(r_bins)
     r_mid = r_bins[:-1] + dr_bins / 2
-    dimensions = positions_n.shape[1]
     surface = 2 * np.pi ** (dimensions / 2) / math.gamma(dimensions / 2)
     shell_volume = surface * r_mid ** (dimensions - 1) * dr_bins
     return r_mid, histogram / shell_volume / number_of_pairs * box_volume

@@ -188,6 +188,18 @@ int dp_compile_check_tuned(int dim, int precision, int ntypes, int even, const c
 int dp_compile_code(int dim, int precision, int ntypes, int even, const char* device_src, int pair_precision,
                     char* out, int64_t out_len, int64_t* code_size);
 
+/* Radial distribution function counts of a single-replica context
+ * (positions.py:58-87, Simulation.py:858-881): counts[i] = number of ordered
+ * pairs (n, m), n == m included, with edges[i] <= r_nm < edges[i+1] (last
+ * bin closed), r_nm the minimum-image distance as NumPy computes it.  The
+ * caller normalises by the shell volumes. */
+int dp_rdf_counts(dp_ctx* ctx, const double* edges, int n_edges, uint64_t* counts);
+
+/* Which kernel the last dp_rdf_counts used: 1 = cell tiles of the current
+ * list (last edge + 2 * max displacement < cell width), 0 = all pairs.  No
+ * counterpart in the reference (its RDF is always all pairs). */
+int dp_get_rdf_path(dp_ctx* ctx, int* cells);
+
 /* Split of the timed step-kernel time: force (or fused) kernel and integrate
  * kernel, each summed over the timed steps (no counterpart in the reference). */
 int dp_get_kernel_breakdown(dp_ctx* ctx, double* force_ms, double* integrate_ms);

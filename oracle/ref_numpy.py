@@ -30,6 +30,10 @@ Pinning (what checks each function; see tests/test_oracle.py):
   arXiv:1303.7011 Eqs. 16-17 with injected noise and the energy-conservation
   bound of tests/test_integrator.py:6-104.
 * ``wrap_into_box`` / ``minimage``: tests/test_positions.py:12-25,78-93.
+* ``rdf_counts``: closed-form shell counts of simple-cubic and fcc lattices
+  (per particle 1 at r = 0, then 6, 12, 8, 6 and 12, 6, 24, 12), the
+  ideal-gas limit g(r) ~ 1 on tests/test_positions.py:66-76's case, and
+  equality with the unblocked positions.get_radial_distribution_function.
 """
 from __future__ import annotations
 
@@ -74,6 +78,25 @@ def generate_positions(unit_cell_coordinates, cells, lattice_constants):
     # reference: positions[...] = ucc + cell_coordinates (ucc first)
     pos = (ucc[None, :, :] + idx[:, None, :]).reshape(-1, ucc.shape[1])
     return pos * lc
+
+
+def rdf_counts(positions, box, r_bins, block=512):
+    """positions.py:58-87 without the normalisation -- the np.histogram
+    counts of the minimum-image distances of every ordered pair, self pairs
+    included: d = x_n - x_m; d -= L * np.round(d / L); r = np.linalg.norm(d)
+    (NumPy 2.2 sums the squares in dimension order).  Rows are processed in
+    blocks; the per-pair arithmetic is the reference's, and histogram counts
+    of blocks add up exactly."""
+    x = np.asarray(positions, dtype=np.float64)
+    r_bins = np.asarray(r_bins, dtype=np.float64)
+    out = np.zeros(r_bins.size - 1, dtype=np.int64)
+    for i0 in range(0, x.shape[0], block):
+        diff = x[i0:i0 + block, np.newaxis, :] - x[np.newaxis, :, :]
+        for i in range(diff.shape[-1]):
+            box_length = np.linalg.norm(box[i])
+            diff[:, :, i] -= box_length * np.round(diff[:, :, i] / box_length)
+        out += np.histogram(np.linalg.norm(diff, axis=2), bins=r_bins)[0]
+    return out
 
 
 def wrap_into_box(positions, image_positions, box):

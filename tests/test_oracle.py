@@ -365,6 +365,44 @@ def test_positions_kats():
     assert np.allclose(x, [[0.1] * 3, [0.5] * 3, [1.9] * 3]) and np.array_equal(img, [[0] * 3, [1] * 3, [-1] * 3])
 
 
+# ---------------------------------------------------------------- RDF
+def test_rdf_counts_lattice_shells():
+    # simple cubic, a = 1, 6^3 cells: per particle 1 (itself), 6, 12, 8, 6
+    x = R.generate_positions(np.array([[0.0, 0, 0]]), (6, 6, 6), (1.0, 1.0, 1.0))
+    n = x.shape[0]
+    c = R.rdf_counts(x, np.array([6.0, 6, 6]), [-0.5, 0.5, 1.2, 1.5, 1.8, 2.1], block=37)
+    assert c.tolist() == [n, 6 * n, 12 * n, 8 * n, 6 * n]
+    # fcc, a = 1.5: 12 at a/sqrt2, 6 at a, 24 at a sqrt(3/2), 12 at a sqrt2
+    fcc = np.array([[0, 0, 0], [.5, .5, 0], [.5, 0, .5], [0, .5, .5]])
+    x = R.generate_positions(fcc, (4, 4, 4), (1.5, 1.5, 1.5))
+    n = x.shape[0]
+    a = 1.5
+    e = [0.9 * a / np.sqrt(2), 1.1 * a / np.sqrt(2), 0.9 * a, 1.1 * a, 0.95 * a * np.sqrt(1.5),
+         1.05 * a * np.sqrt(1.5), 0.95 * a * np.sqrt(2), 1.05 * a * np.sqrt(2)]
+    c = R.rdf_counts(x, np.array([6.0, 6, 6]), e)
+    assert c[0::2].tolist() == [12 * n, 6 * n, 24 * n, 12 * n]
+    assert c[1::2].tolist() == [0, 0, 0]
+    # exact edges: r = 1 falls in [1, 2) and the closed last bin holds r = 2
+    x = R.generate_positions(np.array([[0.0, 0]]), (5, 5), (1.0, 1.0))
+    c = R.rdf_counts(x, np.array([5.0, 5]), [0.0, 1.0, 1.0, 2.0])
+    n = x.shape[0]
+    assert c.tolist() == [n, 0, (4 + 4 + 4) * n]  # r = 1, sqrt2 and 2 (closed)
+
+
+def test_rdf_counts_ideal_gas_and_host_function():
+    from dompap_amd.positions import get_radial_distribution_function, normalise_radial_histogram
+    rng = np.random.default_rng(66)
+    # tests/test_positions.py:66-76's sizes, points spread over the whole box
+    box = np.array([10.0, 10, 10])
+    x = rng.random((1000, 3)) * box
+    r_bins = np.linspace(0.1, 5, 20)
+    c = R.rdf_counts(x, box, r_bins, block=129)
+    r, g = normalise_radial_histogram(c, r_bins, 1000 * 1000, box, 3)
+    r2, g2 = get_radial_distribution_function(x, x, box, r_bins)
+    assert np.array_equal(r, r2) and np.array_equal(g, g2)
+    assert np.all(np.abs(g[r > 0.8] - 1.0) < 0.15)
+
+
 # ---------------------------------------------------------------- RNG
 def test_philox_known_answers():
     # Random123 philox4x32-10 known-answer vectors (Salmon et al., SC'11)
