@@ -47,6 +47,7 @@ struct DpRep {
   double inv_w[DP_DMAX];// nc/L
   double nl_thresh;     // s < nl_thresh  <=>  sqrt(s) < cutoff (exact, FP64)
   u64 nlq_lo, nlq_hi;   // fixed-point r^2 (units^2) below: certainly listed; above: certainly not
+  float nlq_lo_f, nlq_hi_f;  // FP32 screen of the same: a FP32 sum of squares below lo_f is below nlq_lo, above hi_f above nlq_hi
   double skin_half_sq;  // (0.5 * skin)^2
   double dt, T_, tau;   // integrator parameters
   double g1_sq, g1_ca, g1_cb;  // G-JF coefficients for mass 1 (integrator.py:17-40 order, on the host)
@@ -63,6 +64,7 @@ struct DpRep {
 // workgroups serialise, spread ones do not.  The host reduces the slots into
 // max_count / max_cell / max_halo after each read-back.
 #define DP_STAT_SLOTS 64
+#define DP_BLK_MAX 64     // steps per step block, at most
 #define DP_STAT_STRIDE 32
 struct DpCtl {
   u64 step_counter;     // steps taken by the device (RNG counter)
@@ -76,7 +78,15 @@ struct DpCtl {
   i32 max_count;        // host-side reduction of the slots: largest row length
   i32 max_cell;         //   largest cell occupancy at the last rebuild
   i32 max_halo;         //   largest staged halo (records) at the last rebuild
-  i32 pad_[5];
+  i32 blk_done;         // steps the last step block completed
+  i32 pad_[4];
+  // step blocks: the rebuild pipeline runs once at the block's start; step p
+  // of the block runs while no earlier step of the block raised blk_need and
+  // blk_base + p < blk_target
+  u64 blk_base;         // step counter at the block's start
+  u64 blk_target;       // first step index the block must not run
+  u64 blk_left;         // steps the host asks for (set before the block)
+  i32 blk_need[DP_BLK_MAX];  // step p of the block found a particle beyond half the skin
   i32 slot[DP_STAT_SLOTS * DP_STAT_STRIDE];  // [s * stride + 0 / 1 / 2]: the three maxima
 };
 #define DP_STAT_COUNT 0
@@ -140,7 +150,7 @@ struct DpPairArgs {
   u64 seed;
   DpCtl* ctl;
   i32 wrap;               // 1: wrap positions into the box after the update
-  i32 pad1_;
+  i32 blk_pos;            // position of this step in a step block, -1 outside blocks
 };
 
 #ifndef DP_NO_DEVICE_HELPERS
@@ -275,10 +285,28 @@ __device__ __forceinline__ bool dp_skin_trips(double s, double skin_half_sq) {
 
 // Raise ctl->need_rebuild if any lane of the wavefront tripped: a plain store
 // of 1 by one lane, only from wavefronts that tripped (no same-address atomics).
-__device__ __forceinline__ void dp_raise_rebuild(bool trip, DpCtl* ctl) {
+__device__ __forceinline__ void dp_raise_rebuild(bool trip, i32* flag) {
   const u64 b = __ballot(trip);
   if (b != 0ull && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)b) - 1)
-    *(volatile i32*)&ctl->need_rebuild = 1;
+    *(volatile i32*)flag = 1;
+}
+
+// Whether step `pos` of a step block runs (always outside blocks, pos < 0).
+__device__ __forceinline__ bool dp_blk_go(const DpCtl* ctl, int pos) {
+  if (pos < 0) return true;
+  if (ctl->blk_base + (u64)pos >= ctl->blk_target) return false;
+  for (int q = 0; q < pos; ++q)
+    if (ctl->blk_need[q]) return false;
+  return true;
+}
+
+// Step index (noise counter) of the current step, and the flag its skin
+// trigger raises.
+__device__ __forceinline__ u64 dp_step_index(const DpCtl* ctl, int pos) {
+  return pos < 0 ? ctl->step_index : ctl->blk_base + (u64)pos;
+}
+__device__ __forceinline__ i32* dp_need_flag(DpCtl* ctl, int pos) {
+  return pos < 0 ? &ctl->need_rebuild : &ctl->blk_need[pos];
 }
 
 // Replica owning global tile t.

@@ -48,6 +48,57 @@ __global__ void k_prologue(DpCtl* ctl, const DpRep* reps, int R, i64* nl_updates
   }
 }
 
+// Step blocks (a captured graph of `nb` steps with the rebuild pipeline at
+// its start).  k_block_begin takes the place of the per-step prologue for
+// the block's first step: it consumes the rebuild request, clears the
+// block's per-step flags and fixes the block's first step index and the
+// last one it may run (blk_left steps from here, set by the host).
+__global__ void k_block_begin(DpCtl* ctl, i64* nl_updates, int R) {
+  if (blockIdx.x != 0) return;
+  const int tid = threadIdx.x;
+  const int old = ctl->need_rebuild;
+  const int need = old | ctl->force_rebuild;
+  __syncthreads();
+  if (need) {
+    for (int k = tid; k < DP_STAT_SLOTS * DP_STAT_STRIDE; k += blockDim.x) ctl->slot[k] = 0;
+  }
+  for (int k = tid; k < DP_BLK_MAX; k += blockDim.x) ctl->blk_need[k] = 0;
+  if (tid != 0) return;
+  ctl->rebuild = need;
+  ctl->force_rebuild = 0;
+  ctl->need_rebuild = 0;
+  if (old) {
+    for (int r = 0; r < R; ++r) nl_updates[r] += 1;
+  }
+  ctl->blk_base = ctl->step_counter;
+  ctl->blk_target = ctl->step_counter + ctl->blk_left;
+  ctl->blk_done = 0;
+}
+
+// End of a step block: the steps that ran (up to and including the first
+// one that raised its flag, and below the target), the new step counter and
+// the rebuild request for the next step.
+__global__ void k_block_end(DpCtl* ctl, int nb) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int done = 0, need = 0;
+  for (int p = 0; p < nb; ++p) {
+    if (ctl->blk_base + (u64)p >= ctl->blk_target) break;
+    ++done;
+    if (ctl->blk_need[p]) {
+      need = 1;
+      break;
+    }
+  }
+  ctl->blk_done = done;
+  ctl->step_counter = ctl->blk_base + (u64)done;
+  if (done > 0) ctl->step_index = ctl->step_counter - 1;
+  if (need) ctl->need_rebuild = 1;
+}
+
+__global__ void k_block_left(DpCtl* ctl, i64 left) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->blk_left = (u64)left;
+}
+
 __global__ void k_advance_step(DpCtl* ctl) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl->step_index = ctl->step_counter;
@@ -80,7 +131,7 @@ __global__ void k_skin_check(const double* x, const double* xref, const i32* rep
     }
     trip |= dp_skin_trips(s, R.skin_half_sq);
   }
-  dp_raise_rebuild(trip, ctl);
+  dp_raise_rebuild(trip, &ctl->need_rebuild);
 }
 
 // ---------------------------------------------------------------------------
@@ -344,44 +395,72 @@ __device__ __forceinline__ int4 dp_emit_rec(const double* xref, const i32* cellp
   return make_int4(q[0], q[1], q[2], q[3]);
 }
 
-// Squared fixed-point distance (exact: |dq| < 2^31 per dimension, so the sum
-// of D <= 4 squares fits in 64 bits); CM: apply the minimum image along
-// single-cell dimensions.
+// Candidate-to-particle differences along each dimension (the minimum image
+// along single-cell dimensions when CM), the FP32 screen of their squared sum
+// and, for the few candidates the screen cannot decide, the exact one.
 template <int D, bool CM>
-__device__ __forceinline__ u64 dp_emit_s(const int4& a, const int4& b, const DpRep& P) {
-  const i32 av[4] = {a.x, a.y, a.z, a.w};
-  const i32 bv[4] = {b.x, b.y, b.z, b.w};
+__device__ __forceinline__ void dp_emit_dd(const i32* qi, const int4& qj, const DpRep& P, i32* dd) {
+  const i32 bv[4] = {qj.x, qj.y, qj.z, qj.w};
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    i32 t = (i32)((u32)qi[d] - (u32)bv[d]);
+    if (CM && P.nc[d] == 1) {
+      const i32 h = P.Lq[d] >> 1;
+      t = (t > h) ? t - P.Lq[d] : ((t < -h) ? t + P.Lq[d] : t);
+    }
+    dd[d] = t;
+  }
+}
+
+template <int D>
+__device__ __forceinline__ float dp_emit_sf(const i32* dd) {
+  float s = 0.0f;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const float t = (float)dd[d];
+    s = (d == 0) ? t * t : __builtin_fmaf(t, t, s);
+  }
+  return s;
+}
+
+// Exact squared fixed-point distance: |dd| < 2^31 per dimension (the
+// fixed-point scale is chosen so), so a sum of D <= 4 squares fits in 64 bits.
+template <int D>
+__device__ __forceinline__ u64 dp_emit_sq(const i32* dd) {
   u64 s = 0;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    i32 dd = (i32)((u32)av[d] - (u32)bv[d]);
-    if (CM && P.nc[d] == 1) {
-      const i32 h = P.Lq[d] >> 1;
-      dd = (dd > h) ? dd - P.Lq[d] : ((dd < -h) ? dd + P.Lq[d] : dd);
-    }
-    const u32 ad = (u32)(dd < 0 ? -dd : dd);
+    const u32 ad = (u32)(dd[d] < 0 ? -dd[d] : dd[d]);
     s += (u64)ad * (u64)ad;
   }
   return s;
 }
 
+// Emit of one tile's rows.  Every particle of a home cell has the same
+// candidates (its stencil rows, in order).  Wavefront w takes particles w,
+// w + nwaves, ... of each home cell, G at a time; for each chunk of
+// 64 candidates (one per lane) it finds their staged indices and records
+// once, then tests the group's particles against the chunk one after the
+// other (independent chains; the particles' records and running row lengths
+// are wavefront-uniform and live in scalar registers).  A candidate is listed
+// when the FP32 screen puts it below the band; inside the screen's slack the
+// exact fixed-point distance decides, and inside the fixed-point band the
+// reference's FP64 test on xref.
+template <int D> struct DpEmitG { static constexpr int value = D >= 4 ? 4 : 8; };
 template <int D, typename E, bool LDS, bool CM>
 __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const DpRep& P, const double* xref,
                                                   const i32* cellpack, double qinv, const int4* srq, int K, E* nl,
                                                   i32* nlcnt, DpCtl* ctl, int& st_count) {
   constexpr int N3 = DpPow3<D>::value;
   constexpr int NROW = N3 / 3;  // <= 27 for D <= 4: one row per lane
+  constexpr int G = DpEmitG<D>::value;  // particles per candidate pass
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int nwaves = blockDim.x >> 6;
   const i64 Ks = (i64)((K + 7) & ~7);  // row stride: rows are read 8 bytes at a time
   const u64 lo = P.nlq_lo, hi = P.nlq_hi;
-  // Every particle of a home cell has the same candidates (its stencil rows,
-  // in order).  Wavefront w takes particles w, w + nwaves, ... of each home
-  // cell; for each chunk of 64 candidates it finds their staged indices and
-  // records once, then tests each of its particles against the whole chunk
-  // (particle record: LDS broadcast; running row length: one lane each).
+  const float lo_f = P.nlq_lo_f, hi_f = P.nlq_hi_f;
   for (int m = 0; m < P.nhome; ++m) {
     const int uh = S.home_u[m];
     if (uh < 0) continue;
@@ -418,9 +497,22 @@ __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const 
     const int ncand = __shfl(incl, NROW - 1);
     // this wavefront's particles: k = wave + nwaves * j, j = 0 .. npw - 1
     const int npw = (np > wave) ? (np - wave + nwaves - 1) / nwaves : 0;
-    for (int j0 = 0; j0 < npw; j0 += 64) {
-      const int nj = min(64, npw - j0);
-      int cnt = 0;  // lane jj: row length of particle k = wave + nwaves * (j0 + jj)
+    for (int j0 = 0; j0 < npw; j0 += G) {
+      const int nj = min(G, npw - j0);
+      i32 qi[G][4];
+      int selfv[G];
+      u32 cnt[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int k = wave + nwaves * (j0 + min(g, nj - 1));
+        selfv[g] = self0 + k;
+        const int4 r = LDS ? srq[self0 + k] : dp_emit_rec<D>(xref, cellpack, slot0 + k, P, S, uh, qinv);
+        qi[g][0] = __builtin_amdgcn_readfirstlane(r.x);
+        qi[g][1] = __builtin_amdgcn_readfirstlane(r.y);
+        qi[g][2] = __builtin_amdgcn_readfirstlane(r.z);
+        qi[g][3] = __builtin_amdgcn_readfirstlane(r.w);
+        cnt[g] = 0;
+      }
       for (int base = 0; base < ncand; base += 64) {
         const int c = base + lane;
         // row of candidate c: largest r with r_pref[r] <= c (fixed-step search)
@@ -442,40 +534,70 @@ __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const 
           while (S.hbase[uj + 1] <= idx) ++uj;
           qj = LDS ? srq[idx] : dp_emit_rec<D>(xref, cellpack, (i64)S.hstart[uj] + (idx - S.hbase[uj]), P, S, uj, qinv);
         }
-        for (int jj = 0; jj < nj; ++jj) {
-          const int k = wave + nwaves * (j0 + jj);
-          const int self = self0 + k;
-          const i64 slot_i = slot0 + k;
-          const int4 qi = LDS ? srq[self] : dp_emit_rec<D>(xref, cellpack, slot_i, P, S, uh, qinv);
-          const u64 s = dp_emit_s<D, CM>(qi, qj, P);
-          bool hit = valid && (idx != self) && (s < lo);
-          if (valid && idx != self && s >= lo && s <= hi) {
-            // within a few fixed-point units of the cutoff: the reference's test
-            const i64 slot_j = (i64)S.hstart[uj] + (idx - S.hbase[uj]);
-            double ss = 0.0;
+        // screen: bit g of `hit` / `band` for particle g of the group
+        u32 hit = 0u, band = 0u;
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-              const double tt = dp_minimage_lt(dp_sub(xref[slot_i * D + d], xref[slot_j * D + d]), P.L[d], P.halfL[d]);
-              ss = (d == 0) ? dp_mul(tt, tt) : dp_add(ss, dp_mul(tt, tt));
-            }
-            hit = ss < P.nl_thresh;
+        for (int g = 0; g < G; ++g) {
+          if (g < nj) {
+            i32 dd[D];
+            dp_emit_dd<D, CM>(qi[g], qj, P, dd);
+            const float sf = dp_emit_sf<D>(dd);
+            const bool cand_ok = valid && idx != selfv[g];
+            hit |= (cand_ok && sf < lo_f) ? (1u << g) : 0u;
+            band |= (cand_ok && !(sf < lo_f) && sf <= hi_f) ? (1u << g) : 0u;
           }
-          const u64 mask = __ballot(hit);
-          const int c0 = __builtin_amdgcn_readlane(cnt, jj);
-          const int pos = c0 + __popcll(mask & ((1ull << lane) - 1ull));
-          if (hit && pos < K) nl[slot_i * Ks + pos] = (E)idx;
-          if (lane == jj) cnt += __popcll(mask);
+        }
+        if (__ballot(band != 0u) != 0ull) {
+          // the screen's slack: exact fixed-point distance, then within a few
+          // units of the cutoff the reference's test on xref
+          for (int g = 0; g < nj; ++g) {
+            if (band & (1u << g)) {
+              const int k = wave + nwaves * (j0 + g);
+              const int4 r = LDS ? srq[self0 + k] : dp_emit_rec<D>(xref, cellpack, slot0 + k, P, S, uh, qinv);
+              const i32 qg[4] = {r.x, r.y, r.z, r.w};
+              i32 dd[D];
+              dp_emit_dd<D, CM>(qg, qj, P, dd);
+              const u64 sq = dp_emit_sq<D>(dd);
+              bool h = sq < lo;
+              if (!h && sq <= hi) {
+                const i64 slot_i = slot0 + k;
+                const i64 slot_j = (i64)S.hstart[uj] + (idx - S.hbase[uj]);
+                double ss = 0.0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                  const double tt =
+                      dp_minimage_lt(dp_sub(xref[slot_i * D + d], xref[slot_j * D + d]), P.L[d], P.halfL[d]);
+                  ss = (d == 0) ? dp_mul(tt, tt) : dp_add(ss, dp_mul(tt, tt));
+                }
+                h = ss < P.nl_thresh;
+              }
+              if (h) hit |= 1u << g;
+            }
+          }
+        }
+        // rows: the hits of each particle in lane order after its running length
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          if (g < nj) {
+            const bool h = (hit >> g) & 1u;
+            const u64 mask = __ballot(h);
+            const u32 below = __builtin_amdgcn_mbcnt_hi((u32)(mask >> 32), __builtin_amdgcn_mbcnt_lo((u32)mask, 0u));
+            const u32 pos = cnt[g] + below;
+            if (h && pos < (u32)K) nl[(slot0 + (selfv[g] - self0)) * Ks + pos] = (E)idx;
+            cnt[g] += (u32)__popcll(mask);
+          }
         }
       }
-      if (lane < nj) {
-        const int k = wave + nwaves * (j0 + lane);
-        nlcnt[slot0 + k] = cnt;
-        if (cnt > K) ctl->overflow = 1;
-      }
-      int mx = (lane < nj) ? cnt : 0;
 #pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
-      st_count = max(st_count, mx);
+      for (int g = 0; g < G; ++g) {
+        if (g < nj) {
+          if (lane == 0) {
+            nlcnt[slot0 + (selfv[g] - self0)] = (i32)cnt[g];
+            if (cnt[g] > (u32)K) ctl->overflow = 1;
+          }
+          st_count = max(st_count, (int)cnt[g]);
+        }
+      }
     }
   }
 }

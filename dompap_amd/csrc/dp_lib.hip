@@ -225,9 +225,9 @@ struct dp_ctx {
   std::vector<u64> rep_seeds;  // per replica; empty: every replica uses `seed`
 
   // graphs
-  int graph_block = 32;
-  hipGraphExec_t gexec = nullptr;
-  int gexec_integ = 0;
+  int graph_block = 16;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // step-block graphs by starting buffer parity
+  int gexec_integ[2] = {0, 0};
   bool graph_dirty = true;
 
   // kernel timing
@@ -280,8 +280,10 @@ int dalloc(dp_ctx* ctx, T*& p, size_t count) {
 }
 
 int release_graph(dp_ctx* ctx) {
-  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
-  ctx->gexec = nullptr;
+  for (auto& g : ctx->gexec) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
   ctx->graph_dirty = true;
   return DP_OK;
 }
@@ -490,6 +492,19 @@ int refresh_reps(dp_ctx* ctx) {
       const double a = std::max(Rq - margin, 0.0), b = Rq + margin;
       P.nlq_lo = (u64)std::max(std::floor(a * a) - 1.0e4, 0.0);
       P.nlq_hi = (u64)(std::ceil(b * b) + 1.0e4);
+    }
+    // FP32 screen: D converted differences and their sum of squares carry a
+    // relative error below 5 * 2^-24; 2^-20 of slack on each side covers it
+    if (P.nlq_hi == 0) {
+      P.nlq_lo_f = P.nlq_hi_f = -1.0f;  // nothing is listed
+    } else if (P.nlq_lo == 0 && P.nlq_hi == ~(u64)0) {
+      P.nlq_lo_f = -1.0f;  // everything takes the exact path
+      P.nlq_hi_f = INFINITY;
+    } else {
+      float lf = (float)((double)P.nlq_lo * (1.0 - std::ldexp(1.0, -20)));
+      float hf = (float)((double)P.nlq_hi * (1.0 + std::ldexp(1.0, -20)));
+      P.nlq_lo_f = std::nextafter(lf, -INFINITY);
+      P.nlq_hi_f = std::nextafter(hf, INFINITY);
     }
   }
   // entry format: the index of a neighbour in its tile's staged halo; 16
@@ -707,6 +722,7 @@ DpPairArgs pair_args(dp_ctx* ctx, int cur) {
   a.xi = nullptr;
   a.seed = ctx->seed;  // (the kernels key the noise with each replica's DpRep::seed)
   a.ctl = ctx->ctl_d;
+  a.blk_pos = -1;
   a.wrap = 1;
   return a;
 }
@@ -903,15 +919,14 @@ int get_events(dp_ctx* ctx, DpEv3& ev) {
   return DP_OK;
 }
 
-// One step: control, gated rebuild, then the force kernel and the integrate
-// kernel (or the fused kernel).  With `ev`, e[0] / e[1] / e[2] are recorded
-// before the force kernel, between the two and after the integrate kernel
-// (e[1] == e[2] for the fused kernel).
-int launch_one_step(dp_ctx* ctx, int integ, const double* xi_dev, const DpEv3* ev) {
-  CHK(launch_prologue(ctx, 1));
-  CHK(launch_rebuild(ctx));
+// The step kernels: the force kernel and the integrate kernel (or the fused
+// kernel), as step `pos` of a step block (pos < 0: a single step).  With
+// `ev`, e[0] / e[1] / e[2] are recorded before the force kernel, between the
+// two and after the integrate kernel (e[1] == e[2] for the fused kernel).
+int launch_step_kernels(dp_ctx* ctx, int integ, const double* xi_dev, const DpEv3* ev, int pos) {
   DpPairArgs a = pair_args(ctx, ctx->cur);
   a.xi = xi_dev;
+  a.blk_pos = pos;
   if (ev) HIPCHK(hipEventRecord(ev->e[0], ctx->st));
   if (ctx->fused) {
     CHK(launch_pair(ctx, (integ == DP_INTEGRATOR_GJF) ? ctx->f_gjf : ctx->f_lf, a));
@@ -925,6 +940,13 @@ int launch_one_step(dp_ctx* ctx, int integ, const double* xi_dev, const DpEv3* e
   if (ev) HIPCHK(hipEventRecord(ev->e[2], ctx->st));
   ctx->cur = 1 - ctx->cur;
   return DP_OK;
+}
+
+// One step: control, gated rebuild, then the step kernels.
+int launch_one_step(dp_ctx* ctx, int integ, const double* xi_dev, const DpEv3* ev) {
+  CHK(launch_prologue(ctx, 1));
+  CHK(launch_rebuild(ctx));
+  return launch_step_kernels(ctx, integ, xi_dev, ev, -1);
 }
 
 int launch_one_step_timed(dp_ctx* ctx, int integ, const double* xi_dev) {
@@ -953,14 +975,21 @@ int resolve_timing(dp_ctx* ctx) {
   return DP_OK;
 }
 
-int capture_graph(dp_ctx* ctx, int integ) {
-  release_graph(ctx);
+// A step block starting from buffer parity `par`: block control, the gated
+// rebuild pipeline once, `graph_block` gated steps, block end.  Steps after
+// the first one whose skin trigger fired are skipped; the host reads how many
+// ran and launches the next block from the resulting parity.
+int capture_graph(dp_ctx* ctx, int integ, int par) {
+  if (ctx->gexec[par]) (void)hipGraphExecDestroy(ctx->gexec[par]);
+  ctx->gexec[par] = nullptr;
   hipGraph_t g = nullptr;
   const int saved_cur = ctx->cur;
+  ctx->cur = par;
   HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-  int rc = DP_OK;
-  for (int s = 0; s < ctx->graph_block && rc == DP_OK; ++s)
-    rc = launch_one_step(ctx, integ, nullptr, nullptr);
+  k_block_begin<<<1, 256, 0, ctx->st>>>(ctx->ctl_d, ctx->nlupd_d, ctx->R);
+  int rc = launch_rebuild(ctx);
+  for (int s = 0; s < ctx->graph_block && rc == DP_OK; ++s) rc = launch_step_kernels(ctx, integ, nullptr, nullptr, s);
+  k_block_end<<<1, 64, 0, ctx->st>>>(ctx->ctl_d, ctx->graph_block);
   hipError_t e = hipStreamEndCapture(ctx->st, &g);
   ctx->cur = saved_cur;
   if (rc != DP_OK) {
@@ -968,14 +997,13 @@ int capture_graph(dp_ctx* ctx, int integ) {
     return rc;
   }
   if (e != hipSuccess) return ctx->fail(DP_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-  e = hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&ctx->gexec[par], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
-    ctx->gexec = nullptr;
+    ctx->gexec[par] = nullptr;
     return ctx->fail(DP_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
   }
-  ctx->gexec_integ = integ;
-  ctx->graph_dirty = false;
+  ctx->gexec_integ[par] = integ;
   return DP_OK;
 }
 
@@ -1456,17 +1484,25 @@ DP_API int dp_step(dp_ctx* ctx, int64_t nsteps, int integrator, const double* xi
   int64_t left = nsteps;
   // timing brackets every step kernel with events; events recorded inside a
   // captured graph cannot be timed, so timed runs launch directly
-  const bool use_graph = ctx->graph_block > 0 && !ctx->timing && nsteps >= 2 * (int64_t)ctx->graph_block;
+  const bool use_graph = ctx->graph_block > 0 && !ctx->timing && nsteps >= (int64_t)ctx->graph_block / 2;
   if (use_graph) {
-    if (ctx->cur != 0) {
-      CHK(launch_one_step_timed(ctx, integrator, nullptr));
-      --left;
+    if (ctx->graph_dirty) {
+      for (auto& gx : ctx->gexec) {
+        if (gx) (void)hipGraphExecDestroy(gx);
+        gx = nullptr;
+      }
+      ctx->graph_dirty = false;
     }
-    if (ctx->graph_dirty || !ctx->gexec || ctx->gexec_integ != integrator)
-      CHK(capture_graph(ctx, integrator));
-    while (left >= ctx->graph_block) {
-      HIPCHK(hipGraphLaunch(ctx->gexec, ctx->st));
-      left -= ctx->graph_block;
+    while (left > 0) {
+      const int par = ctx->cur;
+      if (!ctx->gexec[par] || ctx->gexec_integ[par] != integrator) CHK(capture_graph(ctx, integrator, par));
+      k_block_left<<<1, 64, 0, ctx->st>>>(ctx->ctl_d, left);
+      HIPCHK(hipGraphLaunch(ctx->gexec[par], ctx->st));
+      CHK(check_ctl(ctx));  // synchronises; the block's step count and errors
+      const int done = ctx->ctl_h->blk_done;
+      if (done <= 0 || done > left) return ctx->fail(DP_ERR_STATE, "step block completed %d steps", done);
+      left -= done;
+      if (done & 1) ctx->cur = 1 - ctx->cur;
     }
   }
   while (left > 0) {
@@ -1732,7 +1768,7 @@ DP_API int dp_get_stats(dp_ctx* ctx, int64_t* nl_updates, int64_t* steps, int32_
 
 DP_API int dp_set_graph_block(dp_ctx* ctx, int steps) {
   if (!ctx || steps < 0) return DP_ERR_ARG;
-  if (steps % 2) ++steps;  // the block must return to the same buffer parity
+  if (steps > DP_BLK_MAX) return ctx->fail(DP_ERR_ARG, "step blocks hold at most %d steps", DP_BLK_MAX);
   ctx->graph_block = steps;
   release_graph(ctx);
   return DP_OK;

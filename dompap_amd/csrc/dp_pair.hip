@@ -485,7 +485,7 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
           // device noise: keyed by the replica's seed and the particle's id in its replica
           if (d < DP_D)
             z_own[j] = a.xi ? a.xi[(i64)pid * DP_D + d]
-                            : dp_normal(P.seed, a.ctl->step_index, (u32)(pid - (u32)P.p0), d);
+                            : dp_normal(P.seed, dp_step_index(a.ctl, a.blk_pos), (u32)(pid - (u32)P.p0), d);
         }
       }
       double sq = 0.0, ca = 0.0, cb = 0.0;
@@ -549,6 +549,7 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
 
 template <int KIND>
 __device__ void dp_tile_run(const DpPairArgs& a) {
+  if (!dp_blk_go(a.ctl, a.blk_pos)) return;  // a skipped step of a step block (uniform)
   __shared__ DpTileSmem<DP_D> S;
   extern __shared__ __align__(16) i32 stage_i[];  // mixed: cap * DP_RS ints; FP64: cap * DP_D doubles + cap types
   const int tid = threadIdx.x;
@@ -594,7 +595,7 @@ __device__ void dp_tile_run(const DpPairArgs& a) {
       dp_tile_particles<KIND, false>(a, S, P, stage_i, spc, trip);
     __syncthreads();
   }
-  if (KIND == 2 || KIND == 3) dp_raise_rebuild(trip, a.ctl);
+  if (KIND == 2 || KIND == 3) dp_raise_rebuild(trip, dp_need_flag(a.ctl, a.blk_pos));
 }
 
 extern "C" __global__ void __launch_bounds__(DP_TILE_THREADS) dp_tile_force(DpPairArgs a) { dp_tile_run<0>(a); }
@@ -611,6 +612,7 @@ extern "C" __global__ void __launch_bounds__(DP_TILE_THREADS) dp_tile_lf(DpPairA
 // ---------------------------------------------------------------------------
 template <int INTEG>
 __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
+  if (!dp_blk_go(a.ctl, a.blk_pos)) return;  // a skipped step of a step block (uniform)
   const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < a.N;
   const i64 ii = valid ? i : 0;
@@ -621,7 +623,7 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
 #pragma clang fp contract(off)
     const double m = a.mass[ii];
     const double dt = P.dt;
-    const u64 step = a.ctl->step_index;
+    const u64 step = dp_step_index(a.ctl, a.blk_pos);
     const u32 pid = (u32)a.orig[ii];
     double xn[DP_D];
     if (INTEG == 1) {
@@ -687,7 +689,7 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
     }
 #endif
   }
-  if (a.wrap) dp_raise_rebuild(valid && dp_skin_trips(sdisp, P.skin_half_sq), a.ctl);
+  if (a.wrap) dp_raise_rebuild(valid && dp_skin_trips(sdisp, P.skin_half_sq), dp_need_flag(a.ctl, a.blk_pos));
 }
 
 extern "C" __global__ void __launch_bounds__(256) dp_integrate_gjf(DpPairArgs a) { dp_integrate_body<1>(a); }

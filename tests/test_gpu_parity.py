@@ -310,6 +310,26 @@ def test_graph_replay_matches_direct_steps():
     assert a.number_of_neighbor_list_updates == b.number_of_neighbor_list_updates
 
 
+def test_step_blocks_of_any_length_match_direct_steps():
+    # step blocks stop at the first skin trigger and at the requested count;
+    # odd block lengths change the buffer parity between blocks
+    for block, n in ((7, 53), (16, 9), (64, 100)):
+        a, x = _lj_system("fp64", cells=5)
+        b, _ = _lj_system("fp64", cells=5)
+        v = np.random.default_rng(block).normal(0, 1.5, size=x.shape)
+        for sim in (a, b):
+            sim.velocities = v
+            sim.set_integrator(time_step=0.005, target_temperature=1.5, temperature_damping_time=1.0)
+        a._device().set_graph_block(block)
+        a.run(n)
+        for _ in range(n):
+            b.step()
+        assert np.array_equal(np.asarray(a.positions), np.asarray(b.positions)), block
+        assert np.array_equal(np.asarray(a.betas), np.asarray(b.betas)), block
+        assert a.number_of_neighbor_list_updates == b.number_of_neighbor_list_updates, block
+        assert a.number_of_steps == b.number_of_steps == n
+
+
 def test_trajectory_parity_fp64_short():
     # 20 G-JF steps with the device noise stream replayed by the oracle
     s, x = _lj_system("fp64", cells=4, density=0.8)
