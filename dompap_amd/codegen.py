@@ -36,14 +36,12 @@ __device__ __forceinline__ float dp_rcp(float x) {
   return fmaf(y, fmaf(-x, y, 1.0f), y);
 }
 #if DP_MODE == 1
-// mixed mode: hardware reciprocal refined by two Newton steps (within an ulp
-// for normal arguments) instead of the correctly rounded quotient
+// mixed mode: hardware reciprocal refined by one Newton step (the relative
+// error e of the estimate becomes e^2) instead of the correctly rounded
+// quotient
 __device__ __forceinline__ double dp_rcp(double x) {
-  double y = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, y, 1.0);
-  y = fma(y, e, y);
-  e = fma(-x, y, 1.0);
-  return fma(y, e, y);
+  const double y = __builtin_amdgcn_rcp(x);
+  return fma(y, fma(-x, y, 1.0), y);
 }
 #else
 __device__ __forceinline__ double dp_rcp(double x) { return 1.0 / x; }
@@ -214,6 +212,27 @@ def _emit(name: str, expr: sp.Expr, var: sp.Symbol) -> str:
     return f"template <class T> __device__ __forceinline__ T {name}(T {var.name}) {{\n  {body}\n}}\n"
 
 
+def _emit_scaled(name: str, expr: sp.Expr, var: sp.Symbol):
+    """Device function ``name<T>(var)`` evaluating ``expr / C`` and the
+    rational C: for a polynomial in 1/var, C is the content of its
+    coefficients (48 q^7 - 24 q^4 becomes 24 * q^4 (2 q^3 - 1), whose inner
+    factor is one FMA with inline constants); C = 1 otherwise.  The pair
+    kernels fold C into their per-type-pair prefactor."""
+    expr = sp.sympify(expr)
+    q = sp.Symbol("dp_q", positive=True)
+    inv = sp.expand(expr.subs(var, 1 / q))
+    if expr.free_symbols == {var} and inv.is_polynomial(q) and not expr.is_polynomial(var):
+        content, prim = sp.Poly(inv, q).primitive()
+        if content != 0 and prim.LC() < 0:
+            content, prim = -content, -prim
+        if content != 0:
+            poly = sp.horner(prim.as_expr(), wrt=q)
+            body = f"const T dp_q = dp_rcp({var.name});\n  return {_PRINTER.doprint(poly)};"
+            src = f"template <class T> __device__ __forceinline__ T {name}(T {var.name}) {{\n  {body}\n}}\n"
+            return src, sp.Rational(content)
+    return _emit(name, expr, var), sp.Integer(1)
+
+
 def _is_even_form(expr_s: sp.Expr, s: sp.Symbol) -> bool:
     for p in sp.preorder_traversal(expr_s):
         if isinstance(p, sp.Pow) and p.base.has(s) and not p.exp.is_Integer:
@@ -259,4 +278,8 @@ def generate(pair_potential_str: str) -> PotentialCode:
     if even:
         parts.append(_emit("dp_pot_gs", g_s, s))
         parts.append(_emit("dp_pot_us", u_s, s))
+        # mixed-mode force term: gs / DP_GS_SCALE (the scale goes into the
+        # kernels' prefactor)
+        src, scale = _emit_scaled("dp_pot_gsn", g_s, s)
+        parts.append(f"#define DP_GS_SCALE ({float(scale)!r})\n" + src)
     return PotentialCode(source="\n".join(parts), even=even, expression=pair_potential_str)

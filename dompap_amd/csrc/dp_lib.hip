@@ -226,8 +226,11 @@ struct dp_ctx {
 
   // graphs
   int graph_block = 16;
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // step-block graphs by starting buffer parity
-  int gexec_integ[2] = {0, 0};
+  hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [timed][starting parity]
+  int gexec_integ[2][2] = {{0, 0}, {0, 0}};
+  std::vector<DpEv3> blk_ev;            // timed step blocks: events of step p (recorded by the graph)
+  bool graph_events_ok = true;          // events recorded inside graphs work on this runtime
+  bool capturing = false;               // a step block is being captured (events become graph nodes)
   bool graph_dirty = true;
 
   // kernel timing
@@ -280,10 +283,11 @@ int dalloc(dp_ctx* ctx, T*& p, size_t count) {
 }
 
 int release_graph(dp_ctx* ctx) {
-  for (auto& g : ctx->gexec) {
-    if (g) (void)hipGraphExecDestroy(g);
-    g = nullptr;
-  }
+  for (auto& row : ctx->gexec)
+    for (auto& g : row) {
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
   ctx->graph_dirty = true;
   return DP_OK;
 }
@@ -927,17 +931,17 @@ int launch_step_kernels(dp_ctx* ctx, int integ, const double* xi_dev, const DpEv
   DpPairArgs a = pair_args(ctx, ctx->cur);
   a.xi = xi_dev;
   a.blk_pos = pos;
-  if (ev) HIPCHK(hipEventRecord(ev->e[0], ctx->st));
+  if (ev) HIPCHK(hipEventRecordWithFlags(ev->e[0], ctx->st, ctx->capturing ? hipEventRecordExternal : 0));
   if (ctx->fused) {
     CHK(launch_pair(ctx, (integ == DP_INTEGRATOR_GJF) ? ctx->f_gjf : ctx->f_lf, a));
-    if (ev) HIPCHK(hipEventRecord(ev->e[1], ctx->st));
+    if (ev) HIPCHK(hipEventRecordWithFlags(ev->e[1], ctx->st, ctx->capturing ? hipEventRecordExternal : 0));
   } else {
     a.F = ctx->F;
     CHK(launch_pair(ctx, ctx->f_force, a));
-    if (ev) HIPCHK(hipEventRecord(ev->e[1], ctx->st));
+    if (ev) HIPCHK(hipEventRecordWithFlags(ev->e[1], ctx->st, ctx->capturing ? hipEventRecordExternal : 0));
     CHK(launch_flat(ctx, (integ == DP_INTEGRATOR_GJF) ? ctx->f_int_gjf : ctx->f_int_lf, a));
   }
-  if (ev) HIPCHK(hipEventRecord(ev->e[2], ctx->st));
+  if (ev) HIPCHK(hipEventRecordWithFlags(ev->e[2], ctx->st, ctx->capturing ? hipEventRecordExternal : 0));
   ctx->cur = 1 - ctx->cur;
   return DP_OK;
 }
@@ -979,17 +983,28 @@ int resolve_timing(dp_ctx* ctx) {
 // rebuild pipeline once, `graph_block` gated steps, block end.  Steps after
 // the first one whose skin trigger fired are skipped; the host reads how many
 // ran and launches the next block from the resulting parity.
-int capture_graph(dp_ctx* ctx, int integ, int par) {
-  if (ctx->gexec[par]) (void)hipGraphExecDestroy(ctx->gexec[par]);
-  ctx->gexec[par] = nullptr;
+int capture_graph(dp_ctx* ctx, int integ, int par, int timed) {
+  hipGraphExec_t& gx = ctx->gexec[timed][par];
+  if (gx) (void)hipGraphExecDestroy(gx);
+  gx = nullptr;
+  if (timed) {
+    while ((int)ctx->blk_ev.size() < ctx->graph_block) {
+      DpEv3 ev;
+      for (auto& e : ev.e) HIPCHK(hipEventCreate(&e));
+      ctx->blk_ev.push_back(ev);
+    }
+  }
   hipGraph_t g = nullptr;
   const int saved_cur = ctx->cur;
   ctx->cur = par;
   HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+  ctx->capturing = true;
   k_block_begin<<<1, 256, 0, ctx->st>>>(ctx->ctl_d, ctx->nlupd_d, ctx->R);
   int rc = launch_rebuild(ctx);
-  for (int s = 0; s < ctx->graph_block && rc == DP_OK; ++s) rc = launch_step_kernels(ctx, integ, nullptr, nullptr, s);
+  for (int s = 0; s < ctx->graph_block && rc == DP_OK; ++s)
+    rc = launch_step_kernels(ctx, integ, nullptr, timed ? &ctx->blk_ev[s] : nullptr, s);
   k_block_end<<<1, 64, 0, ctx->st>>>(ctx->ctl_d, ctx->graph_block);
+  ctx->capturing = false;
   hipError_t e = hipStreamEndCapture(ctx->st, &g);
   ctx->cur = saved_cur;
   if (rc != DP_OK) {
@@ -997,14 +1012,36 @@ int capture_graph(dp_ctx* ctx, int integ, int par) {
     return rc;
   }
   if (e != hipSuccess) return ctx->fail(DP_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-  e = hipGraphInstantiate(&ctx->gexec[par], g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
-    ctx->gexec[par] = nullptr;
+    gx = nullptr;
     return ctx->fail(DP_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
   }
-  ctx->gexec_integ[par] = integ;
+  ctx->gexec_integ[timed][par] = integ;
   return DP_OK;
+}
+
+// Timed step blocks: add the event times of the `done` steps that ran;
+// false if the runtime cannot time events recorded by a graph.
+bool collect_block_times(dp_ctx* ctx, int done) {
+  double t = 0.0, tf = 0.0;
+  for (int p = 0; p < done; ++p) {
+    float ms = 0.f, m1 = 0.f;
+    const DpEv3& ev = ctx->blk_ev[p];
+    if (hipEventElapsedTime(&ms, ev.e[0], ev.e[2]) != hipSuccess ||
+        hipEventElapsedTime(&m1, ev.e[0], ev.e[1]) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    t += ms;
+    tf += m1;
+  }
+  ctx->time_ms += t;
+  ctx->time_force_ms += tf;
+  ctx->time_integ_ms += t - tf;
+  ctx->time_launches += done;
+  return true;
 }
 
 }  // namespace
@@ -1209,6 +1246,8 @@ DP_API int dp_destroy(dp_ctx* ctx) {
   (void)hipSetDevice(ctx->dev);
   if (ctx->st) (void)hipStreamSynchronize(ctx->st);
   release_graph(ctx);
+  for (auto& ev : ctx->blk_ev)
+    for (auto& e : ev.e) (void)hipEventDestroy(e);
   for (auto& ev : ctx->ev_pending)
     for (auto& e : ev.e) (void)hipEventDestroy(e);
   for (auto& ev : ctx->ev_free)
@@ -1484,25 +1523,40 @@ DP_API int dp_step(dp_ctx* ctx, int64_t nsteps, int integrator, const double* xi
   int64_t left = nsteps;
   // timing brackets every step kernel with events; events recorded inside a
   // captured graph cannot be timed, so timed runs launch directly
-  const bool use_graph = ctx->graph_block > 0 && !ctx->timing && nsteps >= (int64_t)ctx->graph_block / 2;
+  const int timed = ctx->timing ? 1 : 0;
+  const bool use_graph = ctx->graph_block > 0 && (!timed || ctx->graph_events_ok) &&
+                         nsteps >= (int64_t)ctx->graph_block / 2;
   if (use_graph) {
     if (ctx->graph_dirty) {
-      for (auto& gx : ctx->gexec) {
-        if (gx) (void)hipGraphExecDestroy(gx);
-        gx = nullptr;
-      }
+      release_graph(ctx);
       ctx->graph_dirty = false;
     }
+    CHK(resolve_timing(ctx));
     while (left > 0) {
       const int par = ctx->cur;
-      if (!ctx->gexec[par] || ctx->gexec_integ[par] != integrator) CHK(capture_graph(ctx, integrator, par));
+      if (!ctx->gexec[timed][par] || ctx->gexec_integ[timed][par] != integrator) {
+        const int rc = capture_graph(ctx, integrator, par, timed);
+        if (rc != DP_OK && timed) {
+          // events inside captured graphs unavailable: time single steps
+          ctx->graph_events_ok = false;
+          (void)hipGetLastError();
+          break;
+        }
+        CHK(rc);
+      }
       k_block_left<<<1, 64, 0, ctx->st>>>(ctx->ctl_d, left);
-      HIPCHK(hipGraphLaunch(ctx->gexec[par], ctx->st));
+      HIPCHK(hipGraphLaunch(ctx->gexec[timed][par], ctx->st));
       CHK(check_ctl(ctx));  // synchronises; the block's step count and errors
       const int done = ctx->ctl_h->blk_done;
       if (done <= 0 || done > left) return ctx->fail(DP_ERR_STATE, "step block completed %d steps", done);
+      const bool times_ok = !timed || collect_block_times(ctx, done);
       left -= done;
       if (done & 1) ctx->cur = 1 - ctx->cur;
+      if (!times_ok) {
+        // events recorded by graphs cannot be timed here: time single steps
+        ctx->graph_events_ok = false;
+        break;
+      }
     }
   }
   while (left > 0) {
@@ -1556,10 +1610,11 @@ DP_API int dp_get_plan(dp_ctx* ctx, int32_t* out, int n) {
   if (!ctx || !out) return DP_ERR_ARG;
   (void)hipSetDevice(ctx->dev);
   CHK(refresh_reps(ctx));
-  const int32_t v[12] = {ctx->tpp,  ctx->nlw, ctx->ob, ctx->qbits, ctx->cap, ctx->cap_emit,
+  const int32_t v[14] = {ctx->tpp,  ctx->nlw, ctx->ob, ctx->qbits, ctx->cap, ctx->cap_emit,
                          (int32_t)ctx->ntiles, ctx->tile_grid, ctx->fused, ctx->reps_h[0].nhome,
-                         ctx->reps_h[0].nhalo, (int32_t)ctx->Ctot};
-  for (int k = 0; k < n && k < 12; ++k) out[k] = v[k];
+                         ctx->reps_h[0].nhalo, (int32_t)ctx->Ctot, ctx->graph_block,
+                         ctx->graph_events_ok ? 1 : 0};
+  for (int k = 0; k < n && k < 14; ++k) out[k] = v[k];
   return DP_OK;
 }
 

@@ -66,6 +66,18 @@ struct DpPC {
   float epsf, sigf;
 };
 
+#ifndef DP_GS_SCALE
+#define DP_GS_SCALE 1.0  // no scaled form generated (odd potentials)
+#endif
+// Mixed mode evaluates the force term as DP_GS_SCALE * dp_pot_gsn(s): the
+// constant goes into ke / kef here.
+#if DP_EVEN
+#define DP_GSN(T, s) dp_pot_gsn<T>(s)
+#define DP_GSK DP_GS_SCALE
+#else
+#define DP_GSK 1.0
+#endif
+
 __device__ __forceinline__ DpPC dp_pc(const DpPairTab& T, int tt) {
   DpPC c;
   c.cut = T.cut_thresh[tt];
@@ -73,11 +85,11 @@ __device__ __forceinline__ DpPC dp_pc(const DpPairTab& T, int tt) {
   c.eps = T.eps[tt];
   c.eos = T.eps_over_sigma[tt];
   c.ks = T.qscale2 * T.inv_sigma2[tt];
-  c.ke = T.eps_over_sigma[tt] * T.qscale;
+  c.ke = T.eps_over_sigma[tt] * T.qscale * DP_GSK;
   c.ghi = T.guard_hi_q[tt];
   c.glo = T.guard_lo_q[tt];
   c.ksf = T.inv_sigma2_qf[tt];
-  c.kef = T.eps_over_sigma_qf[tt];
+  c.kef = T.eps_over_sigma_qf[tt] * (float)DP_GSK;
   c.epsf = T.eps_f[tt];
   c.sigf = T.sigma_f[tt];
   return c;
@@ -293,9 +305,10 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
         }
       }
       double dq[DP_EPL][DP_D];  // exact: 32-bit fixed-point differences
-      float s2[DP_EPL];         // r^2 in fixed-point units^2, rounded once
+      double sd[DP_EPL];        // r^2 in fixed-point units^2 (relative error ~2^-52)
       int tt[DP_EPL];
       bool in[DP_EPL];
+      bool nr[DP_EPL];
       bool near = false;
 #pragma unroll
       for (int c = 0; c < DP_EPL; ++c) {
@@ -310,21 +323,22 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
           }
           dq[c][d] = (double)dd;
         }
-        double sd = dq[c][0] * dq[c][0];
+        sd[c] = dq[c][0] * dq[c][0];
 #pragma unroll
-        for (int d = 1; d < DP_D; ++d) sd = fma(dq[c][d], dq[c][d], sd);
-        s2[c] = (float)sd;
+        for (int d = 1; d < DP_D; ++d) sd[c] = fma(dq[c][d], dq[c][d], sd[c]);
         tt[c] = dp_tt(ti, (DP_NTYPES > 1) ? rj[c].v[DP_D] : 0);
         const DpPC& pc = DP_PC(tt[c]);
-        in[c] = (c < nvalid) && (s2[c] < pc.ghi);  // else certainly beyond the cut
-        near |= in[c] && (s2[c] > pc.glo);
+        // the guards bound the FP32-rounded value; the FP64 one lies closer
+        in[c] = (c < nvalid) && (sd[c] < (double)pc.ghi);  // else certainly beyond the cut
+        nr[c] = in[c] && (sd[c] > (double)pc.glo);
+        near |= nr[c];
       }
       if (near) {
         // too close to the cut to decide from the fixed-point values: decide
         // with the reference's FP64 operations (rare)
 #pragma unroll
         for (int c = 0; c < DP_EPL; ++c) {
-          if (!(in[c] && s2[c] > DP_PC(tt[c]).glo)) continue;
+          if (!nr[c]) continue;
           const i64 slot_j = dp_slot_of(S, nh, idx[c]);
           double s = 0.0;
 #pragma unroll
@@ -340,21 +354,20 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
       for (int c = 0; c < DP_EPL; ++c) {
 #if DP_PAIRPREC == 64
         const DpPC& pc = DP_PC(tt[c]);
-        double s64 = dq[c][0] * dq[c][0];
-#pragma unroll
-        for (int d = 1; d < DP_D; ++d) s64 = fma(dq[c][d], dq[c][d], s64);
-        // entries off the list (padding, beyond the cut) evaluate at s = 1
-        const double sx = in[c] ? s64 * pc.ks : 1.0;
+        // entries off the list (padding: s = 0; beyond the cut) are evaluated
+        // too and their term is selected away (no trap on inf / NaN)
+        const double sx = sd[c] * pc.ks;
 #if DP_EVEN
-        const double g = dp_pot_gs<double>(sx);  // f(x)/x
+        const double gn = DP_GSN(double, sx);  // f(x)/x / DP_GS_SCALE
 #else
         const double xr = sqrt(sx);
-        const double g = dp_pot_f<double>(xr) * dp_rcp(xr);
+        const double gn = dp_pot_f<double>(xr) * dp_rcp(xr);
 #endif
-        const double fs = in[c] ? pc.ke * g : 0.0;  // eps f(x)/r per fixed-point unit
+        const double fs = in[c] ? pc.ke * gn : 0.0;  // eps f(x)/r per fixed-point unit
 #pragma unroll
         for (int d = 0; d < DP_D; ++d) f[d] = fma(fs, dq[c][d], f[d]);
         if (KIND == 1 && in[c]) {
+          const double g = gn * DP_GSK;
 #if DP_EVEN
           e += pc.eps * (dp_pot_us<double>(sx) - T.u_rc);
 #else
@@ -365,18 +378,19 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
         }
 #else
         const DpPC& pc = DP_PC(tt[c]);
-        const float sxf = in[c] ? s2[c] * pc.ksf : 1.0f;
+        const float sxf = (float)sd[c] * pc.ksf;
 #if DP_EVEN
-        const float g = dp_pot_gs<float>(sxf);
+        const float gn = DP_GSN(float, sxf);
 #else
         const float xrf = sqrtf(sxf);
-        const float g = dp_pot_f<float>(xrf) * dp_rcp(xrf);
+        const float gn = dp_pot_f<float>(xrf) * dp_rcp(xrf);
 #endif
         // FP32 pair term, accumulated in FP64 with the exact displacement
-        const double fs = in[c] ? (double)(pc.kef * g) : 0.0;
+        const double fs = in[c] ? (double)(pc.kef * gn) : 0.0;
 #pragma unroll
         for (int d = 0; d < DP_D; ++d) f[d] = fma(fs, dq[c][d], f[d]);
         if (KIND == 1 && in[c]) {
+          const float g = gn * (float)DP_GSK;
 #if DP_EVEN
           e += (double)(pc.epsf * dp_pot_us<float>(sxf)) - (double)pc.epsf * T.u_rc;
 #else

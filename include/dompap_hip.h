@@ -138,7 +138,8 @@ int dp_integrate(dp_ctx* ctx, const double* forces, int integrator, const double
 int dp_set_tuning(dp_ctx* ctx, int threads_per_particle, int tile_particles, int fused);
 /* Current plan: tpp, entry bits, offset bits, fixed-point bits, LDS record
  * capacity (pair, emit), tiles, tile grid, fused, home / halo cells per tile
- * of replica 0, cells (first n values). */
+ * of replica 0, cells, steps per step block, whether timed step blocks record
+ * their events inside the graph (first n values). */
 int dp_get_plan(dp_ctx* ctx, int32_t* out, int n);
 
 /* RNG key and step counter. */
