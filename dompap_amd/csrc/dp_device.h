@@ -48,6 +48,7 @@ struct DpRep {
   double nl_thresh;     // s < nl_thresh  <=>  sqrt(s) < cutoff (exact, FP64)
   u64 nlq_lo, nlq_hi;   // fixed-point r^2 (units^2) below: certainly listed; above: certainly not
   float nlq_lo_f, nlq_hi_f;  // FP32 screen of the same: a FP32 sum of squares below lo_f is below nlq_lo, above hi_f above nlq_hi
+  i32 nlq16_lo, nlq16_hi;    // 16-bit screen (offsets >> 16): s16 < lo is below nlq_lo, s16 > hi above nlq_hi
   double skin_half_sq;  // (0.5 * skin)^2
   double dt, T_, tau;   // integrator parameters
   double g1_sq, g1_ca, g1_cb;  // G-JF coefficients for mass 1 (integrator.py:17-40 order, on the host)

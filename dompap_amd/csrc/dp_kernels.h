@@ -412,6 +412,27 @@ __device__ __forceinline__ void dp_emit_dd(const i32* qi, const int4& qj, const 
   }
 }
 
+// High 16 bits of two fixed-point coordinates packed into one word (x in the
+// low half): the operands of the 16-bit screen.
+typedef short dp_s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u32 dp_pack_hi16(i32 a, i32 b) {
+  return ((u32)a >> 16) | ((u32)b & 0xffff0000u);
+}
+
+// 16-bit screen of a candidate: sum over dimensions of ((a_d >> 16) -
+// (b_d >> 16))^2, each difference wrapping in 16 bits (exact while the
+// displacement is below 2^31 units) and the sum saturating at 2^31 - 1.
+template <int D>
+__device__ __forceinline__ int dp_emit_s16(u32 pxy, u32 pzw, u32 cxy, u32 czw) {
+  const dp_s16x2 dxy = __builtin_bit_cast(dp_s16x2, pxy) - __builtin_bit_cast(dp_s16x2, cxy);
+  int s = __builtin_amdgcn_sdot2(dxy, dxy, 0, true);
+  if (D >= 3) {
+    const dp_s16x2 dzw = __builtin_bit_cast(dp_s16x2, pzw) - __builtin_bit_cast(dp_s16x2, czw);
+    s = __builtin_amdgcn_sdot2(dzw, dzw, s, true);
+  }
+  return s;
+}
+
 template <int D>
 __device__ __forceinline__ float dp_emit_sf(const i32* dd) {
   float s = 0.0f;
@@ -461,6 +482,7 @@ __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const 
   const i64 Ks = (i64)((K + 7) & ~7);  // row stride: rows are read 8 bytes at a time
   const u64 lo = P.nlq_lo, hi = P.nlq_hi;
   const float lo_f = P.nlq_lo_f, hi_f = P.nlq_hi_f;
+  const int lo16 = P.nlq16_lo, hi16 = P.nlq16_hi;
   for (int m = 0; m < P.nhome; ++m) {
     const int uh = S.home_u[m];
     if (uh < 0) continue;
@@ -500,6 +522,7 @@ __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const 
     for (int j0 = 0; j0 < npw; j0 += G) {
       const int nj = min(G, npw - j0);
       i32 qi[G][4];
+      u32 pxy[G], pzw[G];
       int selfv[G];
       u32 cnt[G];
 #pragma unroll
@@ -511,6 +534,8 @@ __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const 
         qi[g][1] = __builtin_amdgcn_readfirstlane(r.y);
         qi[g][2] = __builtin_amdgcn_readfirstlane(r.z);
         qi[g][3] = __builtin_amdgcn_readfirstlane(r.w);
+        pxy[g] = dp_pack_hi16(qi[g][0], qi[g][1]);
+        pzw[g] = dp_pack_hi16(qi[g][2], qi[g][3]);
         cnt[g] = 0;
       }
       for (int base = 0; base < ncand; base += 64) {
@@ -528,28 +553,51 @@ __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const 
         const int u0_row = __shfl(r_u0, row);
         const bool valid = c < ncand;
         const int idx = valid ? lo_row + (c - pf_row) : 0;
+        // halo cell of the candidate: found here for global records, in the
+        // (rare) exact path for staged ones
         int uj = u0_row;
         int4 qj = make_int4(0, 0, 0, 0);
         if (valid) {
-          while (S.hbase[uj + 1] <= idx) ++uj;
-          qj = LDS ? srq[idx] : dp_emit_rec<D>(xref, cellpack, (i64)S.hstart[uj] + (idx - S.hbase[uj]), P, S, uj, qinv);
+          if (LDS) {
+            qj = srq[idx];
+          } else {
+            while (S.hbase[uj + 1] <= idx) ++uj;
+            qj = dp_emit_rec<D>(xref, cellpack, (i64)S.hstart[uj] + (idx - S.hbase[uj]), P, S, uj, qinv);
+          }
         }
         // screen: bit g of `hit` / `band` for particle g of the group
         u32 hit = 0u, band = 0u;
+        if (!CM) {
+          // 16-bit screen: two packed subtractions and two dot products per
+          // particle and candidate
+          const u32 cxy = dp_pack_hi16(qj.x, qj.y), czw = dp_pack_hi16(qj.z, qj.w);
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-          if (g < nj) {
-            i32 dd[D];
-            dp_emit_dd<D, CM>(qi[g], qj, P, dd);
-            const float sf = dp_emit_sf<D>(dd);
-            const bool cand_ok = valid && idx != selfv[g];
-            hit |= (cand_ok && sf < lo_f) ? (1u << g) : 0u;
-            band |= (cand_ok && !(sf < lo_f) && sf <= hi_f) ? (1u << g) : 0u;
+          for (int g = 0; g < G; ++g) {
+            if (g < nj) {
+              const int s16 = dp_emit_s16<D>(pxy[g], pzw[g], cxy, czw);
+              const bool cand_ok = valid && idx != selfv[g];
+              hit |= (cand_ok && s16 < lo16) ? (1u << g) : 0u;
+              band |= (cand_ok && s16 >= lo16 && s16 <= hi16) ? (1u << g) : 0u;
+            }
+          }
+        } else {
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            if (g < nj) {
+              i32 dd[D];
+              dp_emit_dd<D, CM>(qi[g], qj, P, dd);
+              const float sf = dp_emit_sf<D>(dd);
+              const bool cand_ok = valid && idx != selfv[g];
+              hit |= (cand_ok && sf < lo_f) ? (1u << g) : 0u;
+              band |= (cand_ok && !(sf < lo_f) && sf <= hi_f) ? (1u << g) : 0u;
+            }
           }
         }
         if (__ballot(band != 0u) != 0ull) {
           // the screen's slack: exact fixed-point distance, then within a few
           // units of the cutoff the reference's test on xref
+          if (LDS && band != 0u)
+            while (S.hbase[uj + 1] <= idx) ++uj;
           for (int g = 0; g < nj; ++g) {
             if (band & (1u << g)) {
               const int k = wave + nwaves * (j0 + g);
@@ -606,8 +654,10 @@ __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const 
 // table (word 6) and emitted by k_emit_tile_far.  Keeping the global-memory
 // path out of this kernel keeps it at <= 128 VGPRs (4 wavefronts per SIMD).
 template <int D, typename E>
-__global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* xref, const i32* cellpack, double qinv,
-                                                               const i32* cstart, const DpRep* reps, int R,
+__global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* __restrict__ xref,
+                                                               const i32* __restrict__ cellpack, double qinv,
+                                                               const i32* __restrict__ cstart,
+                                                               const DpRep* __restrict__ reps, int R,
                                                                i64 ntiles, int K, int cap, E* nl, i32* nlcnt,
                                                                i32* tiletab, int tstride, DpCtl* ctl, int force) {
   if (!force && !ctl->rebuild) return;
@@ -623,7 +673,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* 
   int st_count = 0, st_cell = 0, st_halo = 0;  // running maxima (per wavefront / thread 0)
   for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
     dp_tile_setup<D>(S, reps, R, cstart, t, cap);
-    const DpRep& P = reps[S.rep];
+    const DpRep& P = reps[__builtin_amdgcn_readfirstlane(S.rep)];  // uniform: scalar loads
     const bool fits = S.fits != 0;
     dp_tile_store<D>(S, P, tiletab + t * (i64)tstride);
     if (tid == 0) tiletab[t * (i64)tstride + 6] = fits ? 0 : 1;
@@ -663,8 +713,9 @@ __global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* 
 // Emit for the tiles marked by k_emit_tile (halo larger than the LDS record
 // area): the same emit reading records from global memory.
 template <int D, typename E>
-__global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile_far(const double* xref, const i32* cellpack,
-                                                                   double qinv, const DpRep* reps, i64 ntiles, int K,
+__global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile_far(const double* __restrict__ xref,
+                                                                   const i32* __restrict__ cellpack, double qinv,
+                                                                   const DpRep* __restrict__ reps, i64 ntiles, int K,
                                                                    E* nl, i32* nlcnt, const i32* tiletab, int tstride,
                                                                    DpCtl* ctl, int force) {
   if (!force && !ctl->rebuild) return;
@@ -675,7 +726,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile_far(const double*
     const i32* tt = tiletab + t * (i64)tstride;
     if (tt[6] == 0) continue;  // uniform over the workgroup
     dp_tile_load<D>(S, tt, 0);
-    const DpRep& P = reps[S.rep];
+    const DpRep& P = reps[__builtin_amdgcn_readfirstlane(S.rep)];  // uniform: scalar loads
     // st_off is not part of the stored tables
     constexpr int n3 = DpPow3<D>::value;
     for (int st = threadIdx.x; st < n3; st += blockDim.x) {
@@ -744,7 +795,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_nl_export(const E* nl, cons
   __shared__ DpTileSmem<D> S;
   for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
     dp_tile_setup<D>(S, reps, R, cstart, t, 0x7fffffff);
-    const DpRep& P = reps[S.rep];
+    const DpRep& P = reps[__builtin_amdgcn_readfirstlane(S.rep)];  // uniform: scalar loads
     for (int k = threadIdx.x; k < S.nhomep; k += blockDim.x) {
       const int m = dp_home_of<D>(S, P.nhome, k);
       const int uh = S.home_u[m];
@@ -921,7 +972,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_rdf_cells(const double* x, 
   unsigned long long pending = 0;
   for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
     dp_tile_load<D>(S, tiletab + t * (i64)tstride, 0);
-    const DpRep& P = reps[S.rep];
+    const DpRep& P = reps[__builtin_amdgcn_readfirstlane(S.rep)];  // uniform: scalar loads
     // a home particle meets at most every staged record of the halo
     const unsigned long long tile_pairs = (unsigned long long)S.nhomep * (unsigned long long)S.nrec;
     if (pending + tile_pairs > DP_RDF_FLUSH_PAIRS) {

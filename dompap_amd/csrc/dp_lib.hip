@@ -510,6 +510,26 @@ int refresh_reps(dp_ctx* ctx) {
       P.nlq_lo_f = std::nextafter(lf, -INFINITY);
       P.nlq_hi_f = std::nextafter(hf, INFINITY);
     }
+    // 16-bit screen: t_d = (a_d >> 16) - (b_d >> 16) is within 1 of the
+    // displacement / 2^16, so with s16 = sum t_d^2 the exact r^2 / 2^32 lies in
+    // [(sqrt(s16) - sqrt(D))^2 - D, (sqrt(s16) + sqrt(D))^2]; 2 units of slack
+    // on each side cover the host's rounding
+    if (P.nlq_hi == 0) {
+      P.nlq16_lo = 0;  // nothing is listed
+      P.nlq16_hi = -1;
+    } else if (P.nlq_lo == 0 && P.nlq_hi == ~(u64)0) {
+      P.nlq16_lo = 0;  // everything takes the exact path (s16 saturates at 2^31 - 1)
+      P.nlq16_hi = 2147483647;
+    } else {
+      const double sD = std::sqrt((double)D);
+      const double qlo = std::ldexp((double)P.nlq_lo, -32), qhi = std::ldexp((double)P.nlq_hi, -32);
+      const double rl = std::sqrt(qlo) - sD;
+      const double lo16 = rl > 0.0 ? std::floor(rl * rl) - 2.0 : 0.0;
+      const double rh = std::sqrt(qhi + D) + sD;
+      const double hi16 = std::ceil(rh * rh) + 2.0;
+      P.nlq16_lo = (i32)std::max(lo16, 0.0);
+      P.nlq16_hi = (i32)std::min(hi16, 2147483647.0);
+    }
   }
   // entry format: the index of a neighbour in its tile's staged halo; 16
   // bits unless a halo could hold more than 2^16 particles

@@ -582,7 +582,7 @@ __device__ void dp_tile_run(const DpPairArgs& a) {
     const i64 t = dp_tile_of(j, a.ntiles);
     if (t >= cend) break;
     dp_tile_load<DP_D>(S, a.tiletab + t * (i64)a.tstride, a.cap);
-    const DpRep& P = a.reps[S.rep];
+    const DpRep& P = a.reps[__builtin_amdgcn_readfirstlane(S.rep)];  // uniform: scalar loads
     const int nh = P.nhalo;
     const bool fits = S.fits != 0;
     if (fits) {
