@@ -90,6 +90,7 @@ struct DpCtl {
   i32 blk_need[DP_BLK_MAX];  // step p of the block found a particle beyond half the skin
   i32 slot[DP_STAT_SLOTS * DP_STAT_STRIDE];  // [s * stride + 0 / 1 / 2]: the three maxima
 };
+static_assert(__builtin_offsetof(DpCtl, blk_need) % 16 == 0, "blk_need is read 16 bytes at a time");
 #define DP_STAT_COUNT 0
 #define DP_STAT_CELL 1
 #define DP_STAT_HALO 2
@@ -293,12 +294,18 @@ __device__ __forceinline__ void dp_raise_rebuild(bool trip, i32* flag) {
 }
 
 // Whether step `pos` of a step block runs (always outside blocks, pos < 0).
+// The earlier steps' flags are read 4 at a time with no early exit, so the
+// loads are independent (scalar loads in kernels that do not write ctl).
 __device__ __forceinline__ bool dp_blk_go(const DpCtl* ctl, int pos) {
   if (pos < 0) return true;
   if (ctl->blk_base + (u64)pos >= ctl->blk_target) return false;
-  for (int q = 0; q < pos; ++q)
-    if (ctl->blk_need[q]) return false;
-  return true;
+  const int4* w = (const int4*)ctl->blk_need;
+  i32 any = 0;
+  for (int q = 0; q < pos; q += 4) {
+    const int4 f = w[q >> 2];
+    any |= f.x | ((q + 1 < pos) ? f.y : 0) | ((q + 2 < pos) ? f.z : 0) | ((q + 3 < pos) ? f.w : 0);
+  }
+  return any == 0;
 }
 
 // Step index (noise counter) of the current step, and the flag its skin

@@ -626,19 +626,32 @@ extern "C" __global__ void __launch_bounds__(DP_TILE_THREADS) dp_tile_lf(DpPairA
 // ---------------------------------------------------------------------------
 template <int INTEG>
 __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
-  if (!dp_blk_go(a.ctl, a.blk_pos)) return;  // a skipped step of a step block (uniform)
   const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < a.N;
   const i64 ii = valid ? i : 0;
-  const int r = a.rep[ii];
+  // the particle's state is loaded first, so that the loads are in flight
+  // while the step gate, the replica parameters and the noise are computed
+  double x0[DP_D], v0[DP_D], f0[DP_D], b0[DP_D], xr0[DP_D];
+#pragma unroll
+  for (int d = 0; d < DP_D; ++d) {
+    x0[d] = a.x[ii * DP_D + d];
+    v0[d] = a.v[ii * DP_D + d];
+    f0[d] = a.F[ii * DP_D + d];
+    b0[d] = (INTEG == 1) ? a.beta[ii * DP_D + d] : 0.0;
+    xr0[d] = a.wrap ? a.xref[ii * DP_D + d] : 0.0;
+  }
+  const double m = a.mass[ii];
+  const u32 pid = (INTEG == 1) ? (u32)a.orig[ii] : 0u;
+#if DP_MODE == 1
+  const u32 pack = a.wrap ? (u32)a.cellpack[ii] : 0u;
+#endif
+  if (!dp_blk_go(a.ctl, a.blk_pos)) return;  // a skipped step of a step block (uniform)
+  const int r = (a.R == 1) ? 0 : a.rep[ii];
   const DpRep& P = a.reps[r];
   double sdisp = 0.0;
   if (valid) {
 #pragma clang fp contract(off)
-    const double m = a.mass[ii];
     const double dt = P.dt;
-    const u64 step = dp_step_index(a.ctl, a.blk_pos);
-    const u32 pid = (u32)a.orig[ii];
     double xn[DP_D];
     if (INTEG == 1) {
       double sq, ca, cb;
@@ -649,6 +662,7 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
 #pragma unroll
         for (int d = 0; d < DP_D; ++d) z[d] = a.xi[(i64)pid * DP_D + d];
       } else {
+        const u64 step = dp_step_index(a.ctl, a.blk_pos);
 #pragma unroll
         for (int k = 0; 2 * k < DP_D; ++k) {
           double z0, z1;
@@ -660,16 +674,16 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
 #pragma unroll
       for (int d = 0; d < DP_D; ++d) {
         const double nb = sq * z[d];
-        const double v1 = dp_gjf_v(ca, cb, dt, m, a.v[ii * DP_D + d], a.F[ii * DP_D + d], a.beta[ii * DP_D + d], nb);
-        xn[d] = a.x[ii * DP_D + d] + v1 * dt;
+        const double v1 = dp_gjf_v(ca, cb, dt, m, v0[d], f0[d], b0[d], nb);
+        xn[d] = x0[d] + v1 * dt;
         a.v[ii * DP_D + d] = v1;
         a.beta[ii * DP_D + d] = nb;
       }
     } else {
 #pragma unroll
       for (int d = 0; d < DP_D; ++d) {
-        const double v1 = dp_lf_v(dt, m, a.v[ii * DP_D + d], a.F[ii * DP_D + d]);
-        xn[d] = a.x[ii * DP_D + d] + dt * v1;
+        const double v1 = dp_lf_v(dt, m, v0[d], f0[d]);
+        xn[d] = x0[d] + dt * v1;
         a.v[ii * DP_D + d] = v1;
       }
     }
@@ -684,14 +698,13 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
       }
       a.x_out[ii * DP_D + d] = xn[d];
       if (a.wrap) {
-        double tq = dp_sub(xn[d], a.xref[ii * DP_D + d]);
+        double tq = dp_sub(xn[d], xr0[d]);
         tq = dp_minimage_gt(tq, P.L[d], P.halfL[d]);
         sdisp = (d == 0) ? dp_mul(tq, tq) : dp_add(sdisp, dp_mul(tq, tq));
       }
     }
 #if DP_MODE == 1
     if (a.wrap) {
-      const u32 pack = (u32)a.cellpack[ii];
 #pragma unroll
       for (int d = 0; d < DP_D; ++d) {
         const u32 c = (pack >> (DP_PACK_BITS * d)) & ((1u << DP_PACK_BITS) - 1u);
