@@ -116,30 +116,68 @@ __device__ __forceinline__ T dp_group_sum(T v) {
 __device__ __forceinline__ i64 dp_row_stride(int K) { return (i64)((K + 7) & ~7); }
 #define DP_EPL (8 / (int)sizeof(dp_ent_t))  // entries per 8-byte load
 
-// Call fn(idx, nvalid) for the 8-byte chunks of a row of length cnt that
-// belong to lane `sub` of its group: entries [EPL * (sub + j * TPP), + EPL)
-// for j = 0, 1, ...; idx[c] for c >= nvalid is unspecified.  The next chunk is
-// loaded while the current one is processed.
+// Row walking: each lane loads DP_CPL chunks of DP_EPL entries (8 bytes per
+// chunk) at a time and keeps DP_PF such loads in flight, so that the latency
+// of the row's loads (the row is streamed from HBM once per step) overlaps
+// the pair arithmetic of the chunks before them.
+#ifndef DP_PF
+#define DP_PF 1
+#endif
+#ifndef DP_CPL
+#define DP_CPL 2
+#endif
+struct DpLoad {
+  u32 w[2 * DP_CPL];
+};
+__device__ __forceinline__ DpLoad dp_ld_entries(const dp_ent_t* p) {
+  DpLoad v;
+#if DP_CPL == 1
+  const uint2 t = *(const uint2*)p;
+  v.w[0] = t.x; v.w[1] = t.y;
+#else
+  const uint4 t = *(const uint4*)p;
+  v.w[0] = t.x; v.w[1] = t.y; v.w[2] = t.z; v.w[3] = t.w;
+#endif
+  return v;
+}
+
+// Call fn(idx, nvalid) for the chunks of a row of length cnt that belong to
+// lane `sub` of its group: loads of DP_CPL chunks at entries
+// [W * (sub + j * TPP), + W), W = DP_EPL * DP_CPL, for j = 0, 1, ...; idx[c]
+// for c >= nvalid is unspecified.
 template <class Fn>
 __device__ __forceinline__ void dp_for_chunks(const dp_ent_t* row, int cnt, int sub, Fn& fn) {
-  const int step = DP_EPL * DP_TPP;
-  int k0 = sub * DP_EPL;
-  uint2 nxt = make_uint2(0u, 0u);
-  if (k0 < cnt) nxt = *(const uint2*)(row + k0);
+  constexpr int W = DP_EPL * DP_CPL;
+  const int step = W * DP_TPP;
+  int k0 = sub * W;
+  DpLoad buf[DP_PF];
+#pragma unroll
+  for (int p = 0; p < DP_PF; ++p) {
+#pragma unroll
+    for (int q = 0; q < 2 * DP_CPL; ++q) buf[p].w[q] = 0u;
+    if (k0 + p * step < cnt) buf[p] = dp_ld_entries(row + k0 + p * step);
+  }
   for (; k0 < cnt; k0 += step) {
-    const uint2 cur = nxt;
-    if (k0 + step < cnt) nxt = *(const uint2*)(row + k0 + step);
-    int idx[DP_EPL];
+    const DpLoad cur = buf[0];
+#pragma unroll
+    for (int p = 0; p + 1 < DP_PF; ++p) buf[p] = buf[p + 1];
+    if (k0 + DP_PF * step < cnt) buf[DP_PF - 1] = dp_ld_entries(row + k0 + DP_PF * step);
+#pragma unroll
+    for (int j = 0; j < DP_CPL; ++j) {
+      const int kj = k0 + j * DP_EPL;
+      if (j > 0 && kj >= cnt) break;
+      int idx[DP_EPL];
 #if DP_NLW == 16
-    idx[0] = (int)(cur.x & 0xffffu);
-    idx[1] = (int)(cur.x >> 16);
-    idx[2] = (int)(cur.y & 0xffffu);
-    idx[3] = (int)(cur.y >> 16);
+      idx[0] = (int)(cur.w[2 * j] & 0xffffu);
+      idx[1] = (int)(cur.w[2 * j] >> 16);
+      idx[2] = (int)(cur.w[2 * j + 1] & 0xffffu);
+      idx[3] = (int)(cur.w[2 * j + 1] >> 16);
 #else
-    idx[0] = (int)cur.x;
-    idx[1] = (int)cur.y;
+      idx[0] = (int)cur.w[2 * j];
+      idx[1] = (int)cur.w[2 * j + 1];
 #endif
-    fn(idx, min(DP_EPL, cnt - k0));
+      fn(idx, min(DP_EPL, cnt - kj));
+    }
   }
 }
 
