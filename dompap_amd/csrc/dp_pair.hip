@@ -662,8 +662,28 @@ extern "C" __global__ void __launch_bounds__(DP_TILE_THREADS) dp_tile_lf(DpPairA
 // this kernel with the wrap, the skin-displacement maximum and the mixed
 // records).
 // ---------------------------------------------------------------------------
+// The wavefront's slice [w0 * D, (w0 + nv) * D) of an [N][D] array from each
+// lane's val[0..D-1], written through LDS so that every store instruction
+// covers consecutive addresses (lane e of store k writes element e + 64 k):
+// whole cache lines instead of a third of each line per instruction.  Called
+// by every lane of the wavefront.
+__device__ __forceinline__ void dp_wave_store(double* dst, i64 w0, int nv, const double (&val)[DP_D], double* buf) {
+  const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+  for (int d = 0; d < DP_D; ++d) buf[lane * DP_D + d] = val[d];
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int k = 0; k < DP_D; ++k) {
+    const int e = lane + 64 * k;
+    const double v = buf[e];
+    if (e < nv * DP_D) dst[w0 * DP_D + e] = v;
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
 template <int INTEG>
 __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
+  __shared__ double wbuf[4][64 * DP_D];  // one slice per wavefront (256 threads)
   const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < a.N;
   const i64 ii = valid ? i : 0;
@@ -682,15 +702,21 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
   const u32 pid = (INTEG == 1) ? (u32)a.orig[ii] : 0u;
 #if DP_MODE == 1
   const u32 pack = a.wrap ? (u32)a.cellpack[ii] : 0u;
+  DpRec rec;  // the record; its slots after the coordinates (type) carry over
+#pragma unroll
+  for (int q = 0; q < DP_RS; ++q) rec.v[q] = 0;
+  if (a.wrap) rec = dp_ld_rec(a.xq + ii * DP_RS);
 #endif
   if (!dp_blk_go(a.ctl, a.blk_pos)) return;  // a skipped step of a step block (uniform)
   const int r = (a.R == 1) ? 0 : a.rep[ii];
   const DpRep& P = a.reps[r];
   double sdisp = 0.0;
+  double xn[DP_D], v1[DP_D], nb[DP_D];
+#pragma unroll
+  for (int d = 0; d < DP_D; ++d) xn[d] = v1[d] = nb[d] = 0.0;
   if (valid) {
 #pragma clang fp contract(off)
     const double dt = P.dt;
-    double xn[DP_D];
     if (INTEG == 1) {
       double sq, ca, cb;
       dp_gjf_coeffs(P, m, sq, ca, cb);
@@ -711,18 +737,15 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
       }
 #pragma unroll
       for (int d = 0; d < DP_D; ++d) {
-        const double nb = sq * z[d];
-        const double v1 = dp_gjf_v(ca, cb, dt, m, v0[d], f0[d], b0[d], nb);
-        xn[d] = x0[d] + v1 * dt;
-        a.v[ii * DP_D + d] = v1;
-        a.beta[ii * DP_D + d] = nb;
+        nb[d] = sq * z[d];
+        v1[d] = dp_gjf_v(ca, cb, dt, m, v0[d], f0[d], b0[d], nb[d]);
+        xn[d] = x0[d] + v1[d] * dt;
       }
     } else {
 #pragma unroll
       for (int d = 0; d < DP_D; ++d) {
-        const double v1 = dp_lf_v(dt, m, v0[d], f0[d]);
-        xn[d] = x0[d] + dt * v1;
-        a.v[ii * DP_D + d] = v1;
+        v1[d] = dp_lf_v(dt, m, v0[d], f0[d]);
+        xn[d] = x0[d] + dt * v1[d];
       }
     }
 #pragma unroll
@@ -733,9 +756,6 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
           a.img[ii * DP_D + d] += (i32)sh;
           xn[d] = xn[d] - sh * P.L[d];
         }
-      }
-      a.x_out[ii * DP_D + d] = xn[d];
-      if (a.wrap) {
         double tq = dp_sub(xn[d], xr0[d]);
         tq = dp_minimage_gt(tq, P.L[d], P.halfL[d]);
         sdisp = (d == 0) ? dp_mul(tq, tq) : dp_add(sdisp, dp_mul(tq, tq));
@@ -747,13 +767,19 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
       for (int d = 0; d < DP_D; ++d) {
         const u32 c = (pack >> (DP_PACK_BITS * d)) & ((1u << DP_PACK_BITS) - 1u);
         const double cen = ((double)c + 0.5) * P.w[d];
-        a.xq_out[ii * DP_RS + d] = (i32)rint(dp_minimage_gt(xn[d] - cen, P.L[d], P.halfL[d]) * a.tab->qinv);
+        rec.v[d] = (i32)rint(dp_minimage_gt(xn[d] - cen, P.L[d], P.halfL[d]) * a.tab->qinv);
       }
-#pragma unroll
-      for (int q = DP_D; q < DP_RS; ++q) a.xq_out[ii * DP_RS + q] = a.xq[ii * DP_RS + q];
+      dp_st_rec(a.xq_out + ii * DP_RS, rec);
     }
 #endif
   }
+  // the wavefront's velocities, betas and positions, whole lines at a time
+  const i64 w0 = i - (threadIdx.x & 63);
+  const int nv = (int)((a.N - w0) < 64 ? (a.N - w0) : 64);
+  double* buf = wbuf[threadIdx.x >> 6];
+  dp_wave_store(a.v, w0, nv, v1, buf);
+  if (INTEG == 1) dp_wave_store(a.beta, w0, nv, nb, buf);
+  dp_wave_store(a.x_out, w0, nv, xn, buf);
   if (a.wrap) dp_raise_rebuild(valid && dp_skin_trips(sdisp, P.skin_half_sq), dp_need_flag(a.ctl, a.blk_pos));
 }
 
