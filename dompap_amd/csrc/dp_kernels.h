@@ -186,11 +186,12 @@ __global__ void k_bin(const double* x, const i32* rep, const DpRep* reps, i64 N,
   }
 }
 
-// Single-workgroup exclusive scan of the cell counts: blocks of 4096 cells,
-// 4 per thread, wavefront scans combined through LDS, a running carry.
-// cstart[c] = sum of the counts before c, cstart[C] = N.  The counts are
-// cleared for the next build as they are read.
+// Single-workgroup exclusive scan of the cell counts: blocks of 16384 cells,
+// 16 per thread (four 16-byte loads), wavefront scans combined through LDS, a
+// running carry.  cstart[c] = sum of the counts before c, cstart[C] = N.  The
+// counts are cleared for the next build as they are read.
 #define DP_SCAN_THREADS 1024
+#define DP_SCAN_PER 16
 __global__ void __launch_bounds__(DP_SCAN_THREADS) k_scan_single(i32* count, i64 C, i32* cstart, i64 N,
                                                                   const DpCtl* ctl) {
   if (!ctl->rebuild) return;
@@ -201,12 +202,22 @@ __global__ void __launch_bounds__(DP_SCAN_THREADS) k_scan_single(i32* count, i64
   const int w = t >> 6;
   constexpr int NW = DP_SCAN_THREADS / 64;
   i32 carry = 0;
-  for (i64 b0 = 0; b0 < C; b0 += 4 * DP_SCAN_THREADS) {
-    const i64 c0 = b0 + 4 * (i64)t;
-    i32 v[4];
+  for (i64 b0 = 0; b0 < C; b0 += DP_SCAN_PER * DP_SCAN_THREADS) {
+    const i64 c0 = b0 + DP_SCAN_PER * (i64)t;
+    i32 v[DP_SCAN_PER];
+    if (c0 + DP_SCAN_PER <= C) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = (c0 + k < C) ? count[c0 + k] : 0;
-    const i32 sthr = v[0] + v[1] + v[2] + v[3];
+      for (int k = 0; k < DP_SCAN_PER; k += 4) {
+        const int4 q = *(const int4*)(count + c0 + k);
+        v[k] = q.x; v[k + 1] = q.y; v[k + 2] = q.z; v[k + 3] = q.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < DP_SCAN_PER; ++k) v[k] = (c0 + k < C) ? count[c0 + k] : 0;
+    }
+    i32 sthr = 0;
+#pragma unroll
+    for (int k = 0; k < DP_SCAN_PER; ++k) sthr += v[k];
     i32 incl = sthr;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -228,12 +239,25 @@ __global__ void __launch_bounds__(DP_SCAN_THREADS) k_scan_single(i32* count, i64
     }
     __syncthreads();
     i32 ex = carry + wsum[w] + (incl - sthr);
+    if (c0 + DP_SCAN_PER <= C) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (c0 + k < C) {
-        cstart[c0 + k] = ex;
-        ex += v[k];
-        count[c0 + k] = 0;
+      for (int k = 0; k < DP_SCAN_PER; k += 4) {
+        int4 o;
+        o.x = ex; ex += v[k];
+        o.y = ex; ex += v[k + 1];
+        o.z = ex; ex += v[k + 2];
+        o.w = ex; ex += v[k + 3];
+        *(int4*)(cstart + c0 + k) = o;
+        *(int4*)(count + c0 + k) = make_int4(0, 0, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < DP_SCAN_PER; ++k) {
+        if (c0 + k < C) {
+          cstart[c0 + k] = ex;
+          ex += v[k];
+          count[c0 + k] = 0;
+        }
       }
     }
     carry += btotal;
@@ -242,25 +266,31 @@ __global__ void __launch_bounds__(DP_SCAN_THREADS) k_scan_single(i32* count, i64
   if (t == 0) cstart[C] = (i32)N;
 }
 
-__global__ void k_scatter(const i32* cell_of, const i32* rank, const i32* cstart, i64 N, i32* tmp,
-                          const DpCtl* ctl) {
+// Particle i to position cstart[cell] + rank in the cell-grouped order, with
+// its original id alongside (keys[pos]) for the in-cell sort.
+__global__ void k_scatter(const i32* cell_of, const i32* rank, const i32* cstart, const i32* orig, i64 N, i32* tmp,
+                          i32* keys, const DpCtl* ctl) {
   if (!ctl->rebuild) return;
-  DP_FOR(i, N) tmp[cstart[cell_of[i]] + rank[i]] = (i32)i;
+  DP_FOR(i, N) {
+    const i32 pos = cstart[cell_of[i]] + rank[i];
+    tmp[pos] = (i32)i;
+    keys[pos] = orig[i];
+  }
 }
 
 // In-cell order by original particle id (deterministic regardless of the
 // atomic ranks): the new position of a member is the number of members of
-// its cell with a smaller id.
-__global__ void k_cellsort(const i32* tmp, const i32* cell_of, const i32* cstart, const i32* orig, i64 N,
-                           i32* perm, const DpCtl* ctl) {
+// its cell with a smaller id, counted over the cell's contiguous keys.
+__global__ void k_cellsort(const i32* tmp, const i32* keys, const i32* cell_of, const i32* cstart, i64 N, i32* perm,
+                           const DpCtl* ctl) {
   if (!ctl->rebuild) return;
   DP_FOR(q, N) {
     const i32 p = tmp[q];
     const i32 c = cell_of[p];
     const i32 lo = cstart[c], hi = cstart[c + 1];
-    const i32 key = orig[p];
+    const i32 key = keys[q];
     i32 rnk = 0;
-    for (i32 k = lo; k < hi; ++k) rnk += (orig[tmp[k]] < key) ? 1 : 0;
+    for (i32 k = lo; k < hi; ++k) rnk += (keys[k] < key) ? 1 : 0;
     perm[lo + rnk] = p;
   }
 }

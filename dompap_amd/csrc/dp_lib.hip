@@ -808,8 +808,11 @@ int launch_rebuild_t(dp_ctx* ctx) {
   k_bin<D><<<gN, DP_BLOCK, 0, st>>>(ctx->x[cur], ctx->rep, ctx->reps_d, N, ctx->cell_of, ctx->rank, ctx->ccount,
                                      ctx->ctl_d);
   k_scan_single<<<1, DP_SCAN_THREADS, 0, st>>>(ctx->ccount, ctx->Ctot, ctx->cstart, N, ctx->ctl_d);
-  k_scatter<<<gN, DP_BLOCK, 0, st>>>(ctx->cell_of, ctx->rank, ctx->cstart, N, ctx->tmp, ctx->ctl_d);
-  k_cellsort<<<gN, DP_BLOCK, 0, st>>>(ctx->tmp, ctx->cell_of, ctx->cstart, ctx->orig, N, ctx->perm, ctx->ctl_d);
+  // the original ids in cell-grouped order go to the head of the scratch
+  // buffer, which k_permute overwrites only after the in-cell sort
+  i32* keys = (i32*)ctx->scratch;
+  k_scatter<<<gN, DP_BLOCK, 0, st>>>(ctx->cell_of, ctx->rank, ctx->cstart, ctx->orig, N, ctx->tmp, keys, ctx->ctl_d);
+  k_cellsort<<<gN, DP_BLOCK, 0, st>>>(ctx->tmp, keys, ctx->cell_of, ctx->cstart, N, ctx->perm, ctx->ctl_d);
   k_permute<D><<<gN, DP_BLOCK, 0, st>>>(ctx->perm, N, ctx->x[cur], ctx->v, ctx->beta, ctx->mass, ctx->img,
                                          ctx->type, ctx->orig, ctx->cell_of, ctx->scratch, ctx->ctl_d);
   k_unpermute_store<D><<<gN, DP_BLOCK, 0, st>>>(N, ctx->x[cur], ctx->v, ctx->beta, ctx->mass, ctx->img, ctx->type,
