@@ -684,11 +684,13 @@ __device__ __forceinline__ void dp_wave_store(double* dst, i64 w0, int nv, const
 template <int INTEG>
 __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
   __shared__ double wbuf[4][64 * DP_D];  // one slice per wavefront (256 threads)
+  // a skipped step of a step block (uniform) returns before loading anything
+  if (!dp_blk_go(a.ctl, a.blk_pos)) return;
   const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < a.N;
   const i64 ii = valid ? i : 0;
   // the particle's state is loaded first, so that the loads are in flight
-  // while the step gate, the replica parameters and the noise are computed
+  // while the replica parameters and the noise are computed
   double x0[DP_D], v0[DP_D], f0[DP_D], b0[DP_D], xr0[DP_D];
 #pragma unroll
   for (int d = 0; d < DP_D; ++d) {
@@ -707,7 +709,6 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
   for (int q = 0; q < DP_RS; ++q) rec.v[q] = 0;
   if (a.wrap) rec = dp_ld_rec(a.xq + ii * DP_RS);
 #endif
-  if (!dp_blk_go(a.ctl, a.blk_pos)) return;  // a skipped step of a step block (uniform)
   const int r = (a.R == 1) ? 0 : a.rep[ii];
   const DpRep& P = a.reps[r];
   double sdisp = 0.0;
