@@ -6,7 +6,7 @@
  * the Simulation method surface and the kernel free functions its tests
  * import.  Each entry point below names the reference interface it replaces
  * (file:line in the dompap repository).  The Python host in dompap_amd/ binds
- * these with ctypes; INTEGRATION.md shows the binding.
+ * these with ctypes (dompap_amd/_lib.py declares every entry point).
  *
  * Conventions
  *  - Every function returns an int status: DP_OK (0) or a negative DP_ERR_*;

@@ -31,7 +31,7 @@ def test_errors_without_device_are_reported():
     lib = _lib.load()
     ctx = ctypes.c_void_p()
     rc = lib.dp_create(ctypes.byref(ctx), 0, 3, 10, 1, 0)
-    if rc == 0:  # a device is present (GPU box): clean up
+    if rc == 0:  # a device is present: clean up
         lib.dp_destroy(ctx)
         return
     assert rc < 0 and lib.dp_last_error(None)

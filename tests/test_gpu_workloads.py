@@ -1,7 +1,6 @@
 """Full-size parity cases for the workloads other than the benchmark's:
-config 2 (3-D binary LJ mixture, N = 262,144, per-type sigma, masses 2:1),
-config 4a (4-D (1-r)**2, N = 2^21) and config 4b (2-D r**-12, N = 2^22),
-all in mixed precision against the C oracle (FP64) on the same positions:
+the 3-D binary LJ mixture (N = 262,144, per-type sigma, masses 2:1), 4-D
+(1-r)**2 (N = 2^21) and 2-D r**-12 (N = 2^22), all in mixed precision against the C oracle (FP64) on the same positions:
 neighbour sets bit-exact, max |dF| / F_rms <= 1e-5, energy and virial
 <= 1e-6 relative; then steps at scale stay finite and the rebuilt lists stay
 exact."""
@@ -44,7 +43,7 @@ def _check_against_oracle(s, pot, r_cut, cutoff, sigma=1.0, types=None):
     assert abs(W - o["virial"]) / abs(o["virial"]) < 1e-6
 
 
-def test_config2_binary_mixture_262k():
+def test_binary_lj_mixture_262k():
     from dompap_amd import Simulation
     rng = np.random.default_rng(2)
     s = Simulation(seed=22, precision="mixed")
@@ -69,7 +68,7 @@ def test_config2_binary_mixture_262k():
     _check_against_oracle(s, "lj", 2.5, cutoff, sigma=sig, types=types)
 
 
-def test_config4a_4d_harmonic_2m():
+def test_4d_harmonic_repulsive_2m():
     from dompap_amd import Simulation
     rng = np.random.default_rng(4)
     s = Simulation(seed=44, precision="mixed")
@@ -88,7 +87,7 @@ def test_config4a_4d_harmonic_2m():
     _check_against_oracle(s, "harm2", 1.0, 1.3)
 
 
-def test_config4b_2d_ipl_4m():
+def test_2d_inverse_power_law_4m():
     from dompap_amd import Simulation
     rng = np.random.default_rng(5)
     s = Simulation(seed=55, precision="mixed")
