@@ -2,7 +2,7 @@
 
 This module is the oracle.  It is not the product: only ``tests/``,
 ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
-import it, and only as the checker.  The product (``dompap_amd``) never
+import it, and only to check results.  The product (``dompap_amd``) never
 imports anything under ``oracle/``.
 
 Every function restates one reference function, with the file:line it follows
