@@ -61,6 +61,46 @@ def workload(name, seed):
         x, box = sc_lattice(16, 0.8)
         T, skin = 0.8, 0.5
         desc = "3D LJ SC 16^3 N=4096 rho=0.8 T=0.8 rc=2.5 skin=0.5 dt=0.005 G-JF tau=10"
+    elif name == "lj-binary-262k":
+        # A/B mixture: sigma_AA = 1, sigma_BB = 1/1.2, sigma_AB their mean,
+        # eps = 1, r_cut = 2.5 sigma_ij (shifted), masses A:B = 2:1, types by a
+        # seeded shuffle of 50/50
+        x, box = sc_lattice(64, 0.75)
+        n = x.shape[0]
+        types = np.zeros(n, dtype=np.int32)
+        types[rng.permutation(n)[: n // 2]] = 1
+        m = np.where(types == 0, 2.0, 1.0)
+        sab = (1.0 + 1.0 / 1.2) / 2.0
+        sig = [[1.0, sab], [sab, 1.0 / 1.2]]
+        v = gaussian_velocities(rng, n, 3, 0.7, m)
+        return dict(name=name, desc="3D binary LJ SC 64^3 N=262144 rho=0.75 T=0.7 rc=2.5 sigma_ij skin=0.5 "
+                    "dt=0.005 G-JF tau=10 masses 2:1", x=x, v=v, m=m, box=box, pot="4*(r**-12-r**-6)", r_cut=2.5,
+                    skin=0.5, dt=0.005, T=0.7, tau=10.0, K=192, seed=seed, types=types, sigma=sig)
+    elif name == "harmonic-4d-2m":
+        # 4-D (1-r)^2, r_cut = 1, density 1.2 on a 32x32x32x64 hypercubic
+        # lattice perturbed by N(0, (0.1 a)^2)
+        a = (1.0 / 1.2) ** 0.25
+        from dompap_amd.positions import generate_positions
+        x = generate_positions(np.zeros((1, 4)), (32, 32, 32, 64), (a,) * 4)
+        box = np.array([32 * a, 32 * a, 32 * a, 64 * a])
+        x = (x + rng.normal(0.0, 0.1 * a, size=x.shape)) % box
+        n = x.shape[0]
+        m = np.ones(n)
+        v = gaussian_velocities(rng, n, 4, 0.001, m)
+        return dict(name=name, desc="4D (1-r)^2 hypercubic 32x32x32x64 N=2097152 rho=1.2 T=0.001 rc=1 skin=0.3 "
+                    "dt=0.005 G-JF tau=1", x=x, v=v, m=m, box=box, pot="(1-r)**2", r_cut=1.0, skin=0.3, dt=0.005,
+                    T=0.001, tau=1.0, K=64, seed=seed)
+    elif name == "ipl-2d-4m":
+        # 2-D r^-12, r_cut = 1.5, density 1 on a 2048^2 square lattice
+        # perturbed by N(0, 0.05^2)
+        x, box = sc_lattice(2048, 1.0, dim=2)
+        x = (x + rng.normal(0.0, 0.05, size=x.shape)) % box
+        n = x.shape[0]
+        m = np.ones(n)
+        v = gaussian_velocities(rng, n, 2, 1.0, m)
+        return dict(name=name, desc="2D r^-12 square 2048^2 N=4194304 rho=1 T=1 rc=1.5 skin=0.3 dt=0.002 G-JF tau=1",
+                    x=x, v=v, m=m, box=box, pot="r**-12", r_cut=1.5, skin=0.3, dt=0.002, T=1.0, tau=1.0, K=32,
+                    seed=seed)
     else:
         raise SystemExit(f"unknown workload {name}")
     n = x.shape[0]
@@ -77,8 +117,10 @@ def make_engine(wl, precision, pair_precision="fp64", tuning=None):
     eng = Engine(dim, n, 1, precision)
     eng.set_box(wl["box"])
     dp = device_potential(wl["pot"], wl["r_cut"])
-    eng.load_potential(dp.code, dp.r_cut, dp.u_rc, [[1.0]], [[1.0]], pair_precision)
-    eng.set_neighbor_params(1.0 * wl["r_cut"] + wl["skin"], wl["skin"], wl["K"])
+    sig = np.asarray(wl.get("sigma", [[1.0]]), dtype=np.float64)
+    eng.load_potential(dp.code, dp.r_cut, dp.u_rc, sig, np.ones_like(sig), pair_precision)
+    # list cutoff: largest sigma times r_cut, plus the skin (Simulation.py:353-356)
+    eng.set_neighbor_params(float(sig.max()) * wl["r_cut"] + wl["skin"], wl["skin"], wl["K"])
     eng.set_integrator(wl["dt"], wl["T"], wl["tau"])
     eng.set_seed(wl["seed"], 0)
     if tuning is not None:
@@ -86,6 +128,8 @@ def make_engine(wl, precision, pair_precision="fp64", tuning=None):
     eng.set_field("positions", wl["x"])
     eng.set_field("velocities", wl["v"])
     eng.set_field("masses", wl["m"])
+    if "types" in wl:
+        eng.set_field("types", wl["types"])
     eng.build_neighbor_list()
     return eng
 
@@ -220,7 +264,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", default="lj-1m")
+    ap.add_argument("--workload", default="lj-1m",
+                    choices=["lj-1m", "lj-4k", "lj-binary-262k", "harmonic-4d-2m", "ipl-2d-4m"])
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--pair-precision", default="fp64", choices=["fp64", "fp32"],
                     help="mixed mode: precision of the in-range pair terms")
