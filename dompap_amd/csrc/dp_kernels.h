@@ -680,6 +680,196 @@ __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const 
   }
 }
 
+// Exact decision for a candidate inside the 16-bit screen's slack (rare; kept
+// out of line so that its operands do not hold registers in the screen loop):
+// the exact fixed-point distance, then within a few units of the cutoff the
+// reference's test on xref.  si / sj: staged indices; slot_i: the particle's slot.
+template <int D>
+__device__ __attribute__((noinline)) bool dp_emit_exact(const DpTileSmem<D>& S, const DpRep& P, const double* xref,
+                                                       const int4* srq, int si, int sj, i64 slot_i) {
+  const int4 r = srq[si];
+  const i32 qg[4] = {r.x, r.y, r.z, r.w};
+  i32 dd[D];
+  dp_emit_dd<D, false>(qg, srq[sj], P, dd);
+  const u64 sq = dp_emit_sq<D>(dd);
+  if (sq < P.nlq_lo) return true;
+  if (sq > P.nlq_hi) return false;
+  const int uj = dp_halo_of<D>(S, P.nhalo, sj);
+  const i64 slot_j = (i64)S.hstart[uj] + (sj - S.hbase[uj]);
+  double ss = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double tt = dp_minimage_lt(dp_sub(xref[slot_i * D + d], xref[slot_j * D + d]), P.L[d], P.halfL[d]);
+    ss = (d == 0) ? dp_mul(tt, tt) : dp_add(ss, dp_mul(tt, tt));
+  }
+  return ss < P.nl_thresh;
+}
+
+// One home cell's table pass: its tables and wavefront-uniform values.
+template <int D, typename E> struct DpEmitCell {
+  const DpTileSmem<D>& S;
+  const DpRep& P;
+  const double* xref;
+  const int4* srq;
+  const u16* ctab;
+  E* nl;
+  i32* nlcnt;
+  DpCtl* ctl;
+  int K;
+  i64 Ks;
+  int self0;
+  i64 slot0;
+  int ncand, cs0, lo16, hi16;
+};
+
+// Rows of NJ particles k0, k0 + NW, .. of one home cell against its candidate
+// table; returns the longest row.  Per chunk of 64 candidates and particle:
+// the 16-bit screen (lane masks of single compares, combined in scalar code),
+// then the hits' stores; a candidate in the screen's slack (rare) sends that
+// particle's chunk through the exact decision first.
+template <int D, typename E, int NJ>
+__device__ __forceinline__ int dp_emit_group(const DpEmitCell<D, E>& C, int k0) {
+  constexpr int NW = DP_TILE_THREADS / 64;
+  const int lane = threadIdx.x & 63;
+  u32 pxy[NJ], pzw[NJ], cnt[NJ];
+#pragma unroll
+  for (int g = 0; g < NJ; ++g) {
+    const int4 r = C.srq[C.self0 + k0 + NW * g];
+    pxy[g] = __builtin_amdgcn_readfirstlane(dp_pack_hi16(r.x, r.y));
+    pzw[g] = __builtin_amdgcn_readfirstlane(dp_pack_hi16(r.z, r.w));
+    cnt[g] = 0;
+  }
+  const u32 row0 = (u32)((C.slot0 + k0) * C.Ks);  // nl index of the group's first row
+  const u32 rstep = (u32)(NW * C.Ks);
+  for (int base = 0; base < C.ncand; base += 64) {
+    const int c = base + lane;
+    const bool valid = c < C.ncand;
+    const int idx = valid ? (int)C.ctab[c] : C.self0;
+    const int4 qj = C.srq[idx];
+    const u32 cxy = dp_pack_hi16(qj.x, qj.y), czw = dp_pack_hi16(qj.z, qj.w);
+    const u64 vmask = __builtin_amdgcn_ballot_w64(valid);
+    // row offsets are recomputed per chunk in scalar code rather than held
+    // in registers across the loop
+    u32 rs = rstep;
+    asm volatile("" : "+s"(rs));
+#pragma unroll
+    for (int g = 0; g < NJ; ++g) {
+      const int k = k0 + NW * g;
+      const int s16 = dp_emit_s16<D>(pxy[g], pzw[g], cxy, czw);
+      const bool lt = s16 < C.lo16;
+      const bool ns = c != C.cs0 + k;
+      const u64 ltm = __builtin_amdgcn_ballot_w64(lt);
+      const u64 lem = __builtin_amdgcn_ballot_w64(s16 <= C.hi16);
+      const u32 rowg = row0 + (u32)g * rs;
+      if ((lem & ~ltm) == 0ull) {
+        // every candidate decided by the screen
+        const u64 hm = ltm & __builtin_amdgcn_ballot_w64(ns) & vmask;
+        const u32 pos = __builtin_amdgcn_mbcnt_hi((u32)(hm >> 32), __builtin_amdgcn_mbcnt_lo((u32)hm, cnt[g]));
+        if (valid && ns && lt && pos < (u32)C.K) C.nl[rowg + pos] = (E)idx;
+        cnt[g] += (u32)__popcll(hm);
+      } else {
+        // candidates in the screen's slack (rare): the exact decision
+        bool h = valid && ns && lt;
+        if (valid && ns && !lt && s16 <= C.hi16)
+          h = dp_emit_exact<D>(C.S, C.P, C.xref, C.srq, C.self0 + k, idx, C.slot0 + k);
+        const u64 hm = __builtin_amdgcn_ballot_w64(h);
+        const u32 pos = __builtin_amdgcn_mbcnt_hi((u32)(hm >> 32), __builtin_amdgcn_mbcnt_lo((u32)hm, cnt[g]));
+        if (h && pos < (u32)C.K) C.nl[rowg + pos] = (E)idx;
+        cnt[g] += (u32)__popcll(hm);
+      }
+    }
+  }
+  int mx = 0;
+#pragma unroll
+  for (int g = 0; g < NJ; ++g) {
+    if (lane == 0) {
+      C.nlcnt[C.slot0 + k0 + NW * g] = (i32)cnt[g];
+      if (cnt[g] > (u32)C.K) C.ctl->overflow = 1;
+    }
+    mx = max(mx, (int)cnt[g]);
+  }
+  return mx;
+}
+
+// Emit of one tile's rows (up to 8 particles per candidate pass) for tiles staged in LDS with no single-cell
+// dimension.  Same rows, in the same order, as dp_emit_particles.  Per home
+// cell, the workgroup first writes the candidate list (staged indices, in
+// stencil-row order) to an LDS table `ctab`, so that a lane finds its
+// candidate with one LDS read.  The group's particle index, row pointer and
+// running row length are wavefront-uniform (scalar registers); the screen's
+// results stay lane masks: a hit is stored under its own compare, at
+// mbcnt(mask) after the running length.  The self entry sits at a known
+// candidate position (the home cell is the centre of its stencil row).
+template <int D, typename E>
+__device__ __forceinline__ void dp_emit_particles_tab(const DpTileSmem<D>& S, const DpRep& P, const double* xref,
+                                                      const int4* srq, u16* ctab, int K, E* nl, i32* nlcnt,
+                                                      DpCtl* ctl, int& st_count) {
+  constexpr int N3 = DpPow3<D>::value;
+  constexpr int NROW = N3 / 3;
+  constexpr int G = 8;
+  constexpr int NW = DP_TILE_THREADS / 64;
+  constexpr int RC = (N3 / 2) / 3;  // stencil row holding the centre cell (its middle cell)
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const i64 Ks = (i64)((K + 7) & ~7);
+  const int lo16 = P.nlq16_lo, hi16 = P.nlq16_hi;
+  for (int m = 0; m < P.nhome; ++m) {
+    // (LDS values that are the same in every lane: read into scalar registers)
+    const int uh = __builtin_amdgcn_readfirstlane(S.home_u[m]);
+    if (uh < 0) continue;  // uniform over the workgroup: no barrier is skipped by part of it
+    const int self0 = __builtin_amdgcn_readfirstlane(S.hbase[uh]);
+    const int np = __builtin_amdgcn_readfirstlane(S.hbase[uh + 1]) - self0;
+    const i64 slot0 = __builtin_amdgcn_readfirstlane(S.hstart[uh]);
+    // lane r < NROW: stencil row r = cells 3r .. 3r + 2, consecutive halo cells
+    int r_lo = 0, r_len = 0;
+    if (lane < NROW) {
+      const int u0 = uh + S.st_off[3 * lane];
+      r_lo = S.hbase[u0];
+      r_len = S.hbase[uh + S.st_off[3 * lane + 2] + 1] - r_lo;
+    }
+    int incl = r_len;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int vv = __shfl_up(incl, off);
+      if (lane >= off) incl += vv;
+    }
+    const int r_pref = incl - r_len;
+    const int ncand = __builtin_amdgcn_readlane(incl, NROW - 1);
+    // candidate position of the home cell's first particle
+    const int cs0 = __builtin_amdgcn_readlane(r_pref, RC) + (self0 - __builtin_amdgcn_readlane(r_lo, RC));
+    // the table: row r's staged indices at r_pref[r] ..; wavefront w writes rows w, w + NW, ..
+    for (int r = wave; r < NROW; r += NW) {
+      const int rl = __builtin_amdgcn_readlane(r_lo, r);
+      const int rn = __builtin_amdgcn_readlane(r_len, r);
+      const int rp = __builtin_amdgcn_readlane(r_pref, r);
+      for (int i = lane; i < rn; i += 64) ctab[rp + i] = (u16)(rl + i);
+    }
+    __syncthreads();
+    // this wavefront's particles: k = wave + NW * j, j = 0 .. npw - 1, in
+    // groups of up to G (the group size is a template argument of the pass)
+    const int npw = (np > wave) ? (np - wave + NW - 1) / NW : 0;
+    for (int j0 = 0; j0 < npw; j0 += G) {
+      const int nj = min(G, npw - j0);
+      const int k0 = wave + NW * j0;
+      const DpEmitCell<D, E> C{S, P, xref, srq, ctab, nl, nlcnt, ctl, K, Ks, self0, slot0, ncand, cs0, lo16, hi16};
+      int c = 0;
+      switch (nj) {
+        case 1: c = dp_emit_group<D, E, 1>(C, k0); break;
+        case 2: c = dp_emit_group<D, E, 2>(C, k0); break;
+        case 3: c = dp_emit_group<D, E, 3>(C, k0); break;
+        case 4: c = dp_emit_group<D, E, 4>(C, k0); break;
+        case 5: c = dp_emit_group<D, E, (G >= 5 ? 5 : 1)>(C, k0); break;
+        case 6: c = dp_emit_group<D, E, (G >= 6 ? 6 : 1)>(C, k0); break;
+        case 7: c = dp_emit_group<D, E, (G >= 7 ? 7 : 1)>(C, k0); break;
+        default: c = dp_emit_group<D, E, (G >= 8 ? 8 : 1)>(C, k0); break;
+      }
+      st_count = max(st_count, c);
+    }
+    __syncthreads();  // the next home cell rewrites the table
+  }
+}
+
 // Tiles whose halo fits the LDS record area; the others are marked in their
 // table (word 6) and emitted by k_emit_tile_far.  Keeping the global-memory
 // path out of this kernel keeps it at <= 128 VGPRs (4 wavefronts per SIMD).
@@ -695,7 +885,8 @@ __global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* 
   // unbounded) emit; the error is reported after the step block
   if (ctl->nonfinite) return;
   __shared__ DpTileSmem<D> S;
-  extern __shared__ __align__(16) int4 srq[];  // cap staged records
+  extern __shared__ __align__(16) int4 srq[];  // cap staged records, then cap u16 candidate indices
+  u16* ctab = (u16*)(srq + cap);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -723,7 +914,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* 
     const bool cm = P.collapsed_mask != 0;
     if (fits) {
       if (cm) dp_emit_particles<D, E, true, true>(S, P, xref, cellpack, qinv, srq, K, nl, nlcnt, ctl, st_count);
-      else dp_emit_particles<D, E, true, false>(S, P, xref, cellpack, qinv, srq, K, nl, nlcnt, ctl, st_count);
+      else dp_emit_particles_tab<D, E>(S, P, xref, srq, ctab, K, nl, nlcnt, ctl, st_count);
     }
     __syncthreads();
   }

@@ -566,7 +566,8 @@ int refresh_reps(dp_ctx* ctx) {
     ce = std::min<i64>(ce, (i64)(48 * 1024 / 16));
     ctx->cap_emit = (int)std::max<i64>(ce, 64);
   }
-  ctx->lds_emit = (size_t)ctx->cap_emit * 16;  // one int4 record per staged particle
+  // one int4 record and one u16 candidate-table entry per staged particle
+  ctx->lds_emit = (size_t)ctx->cap_emit * 16 + ((size_t)ctx->cap_emit * 2 + 15) / 16 * 16;
   ctx->ntiles = t0;
   {
     i64 g = std::min<i64>(t0, 4096);
@@ -601,7 +602,11 @@ int refresh_reps(dp_ctx* ctx) {
     }
   }
   // rows padded to a multiple of 8 entries (the pair kernels read 8 bytes at a time)
-  const size_t nl_bytes = (size_t)ctx->N * (size_t)((ctx->K + 7) & ~7) * (size_t)(nlw / 8);
+  const size_t nl_entries = (size_t)ctx->N * (size_t)((ctx->K + 7) & ~7);
+  if (nl_entries >= ((size_t)1 << 32))  // the emit indexes the list with 32-bit entry offsets
+    return ctx->fail(DP_ERR_ARG, "neighbour list of %zu entries exceeds 2^32 - 1 (N x max_number_of_neighbors)",
+                     nl_entries);
+  const size_t nl_bytes = nl_entries * (size_t)(nlw / 8);
   if (nl_bytes != ctx->nl_bytes || !ctx->nl) {
     dfree(ctx->nl);
     ctx->nl_bytes = 0;
