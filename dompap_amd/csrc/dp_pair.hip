@@ -284,6 +284,11 @@ __device__ __forceinline__ i64 dp_slot_of(const DpTileSmem<DP_D>& S, int nh, int
   return (i64)S.hstart[u] + (idx - S.hbase[u]);
 }
 
+// Halo records staged per thread and batch (mixed mode).
+#ifndef DP_SB
+#define DP_SB 3
+#endif
+
 // Per-particle pass over the home particles of the staged tile.  LDS selects
 // where neighbour data come from: the staged halo (LDS = true) or global
 // memory through the tile tables (tiles whose halo exceeds the LDS capacity).
@@ -625,9 +630,30 @@ __device__ void dp_tile_run(const DpPairArgs& a) {
     const bool fits = S.fits != 0;
     if (fits) {
 #if DP_MODE == 1
-      for (int idx = tid; idx < S.nrec; idx += blockDim.x) {
-        const int u = dp_halo_of<DP_D>(S, nh, idx);
-        dp_st_rec(stage_i + idx * DP_RS, dp_halo_rec(a, S, P, u, (i64)S.hstart[u] + (idx - S.hbase[u])));
+      // DP_SB records per thread at a time: their loads are all issued before
+      // the first one is waited for
+      for (int b0 = tid; b0 < S.nrec; b0 += DP_SB * DP_TILE_THREADS) {
+        DpRec r[DP_SB];
+        u32 sh[DP_SB][DP_D];
+#pragma unroll
+        for (int q = 0; q < DP_SB; ++q) {
+          const int idx = b0 + q * DP_TILE_THREADS;
+          if (idx < S.nrec) {
+            const int u = dp_halo_of<DP_D>(S, nh, idx);
+            r[q] = dp_ld_rec(a.xq + ((i64)S.hstart[u] + (idx - S.hbase[u])) * DP_RS);
+#pragma unroll
+            for (int d = 0; d < DP_D; ++d) sh[q][d] = dp_halo_shift<DP_D>(S, P, u, d);
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < DP_SB; ++q) {
+          const int idx = b0 + q * DP_TILE_THREADS;
+          if (idx < S.nrec) {
+#pragma unroll
+            for (int d = 0; d < DP_D; ++d) r[q].v[d] = (i32)((u32)r[q].v[d] + sh[q][d]);
+            dp_st_rec(stage_i + idx * DP_RS, r[q]);
+          }
+        }
       }
 #else
       double* sxd = (double*)stage_i;
