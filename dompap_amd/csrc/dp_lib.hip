@@ -347,8 +347,12 @@ int refresh_reps(dp_ctx* ctx) {
   // home particles per tile: one pass of the workgroup's lane groups
   const int target = ctx->tune_tile > 0 ? ctx->tune_tile : DP_TILE_THREADS / tpp;
   // LDS record capacity of the pair kernels (48 KB of records)
-  // staged record: mixed RS ints; FP64 D doubles (+ the type with several types)
-  const size_t rec_bytes = (ctx->mode == DP_MIXED) ? (size_t)ctx->rs * 4
+  // staged record: mixed, the coordinates (+ the type with several types) in
+  // a power of two of ints (DP_LS of the pair kernels); FP64, D doubles (+ the
+  // type with several types)
+  int ls = 1;
+  while (ls < D + (ctx->ntypes > 1 ? 1 : 0)) ls *= 2;
+  const size_t rec_bytes = (ctx->mode == DP_MIXED) ? (size_t)ls * 4
                                                    : (size_t)D * 8 + (ctx->ntypes > 1 ? 4 : 0);
   const i64 lds_rec_max = (i64)(48 * 1024 / rec_bytes);
   i64 c0 = 0, t0 = 0;
@@ -1377,7 +1381,7 @@ DP_API int dp_load_potential(dp_ctx* ctx, const char* device_src, int even, doub
   release_graph(ctx);
   DpPairTab& T = ctx->tab_h;
   std::memset(&T, 0, sizeof(T));
-  if (ntypes != ctx->ntypes) ctx->reps_dirty = true;  // FP64 records carry the type with several types
+  if (ntypes != ctx->ntypes) ctx->reps_dirty = true;  // staged records carry the type with several types
   ctx->ntypes = ntypes;
   for (int a = 0; a < ntypes; ++a)
     for (int b = 0; b < ntypes; ++b) {

@@ -268,6 +268,41 @@ __device__ __forceinline__ void dp_st_rec(i32* p, const DpRec& r) {
 #endif
 }
 
+// Staged (LDS) records: the coordinates, and the type only with several
+// types, in DP_LS ints (a power of two, at most DP_RS).  With one type this
+// halves the record for D = 2 and D = 4 (the HBM record keeps its type slot).
+#define DP_LS ((DP_D + (DP_NTYPES > 1 ? 1 : 0)) <= 1 ? 1 : (DP_D + (DP_NTYPES > 1 ? 1 : 0)) <= 2 ? 2 : (DP_D + (DP_NTYPES > 1 ? 1 : 0)) <= 4 ? 4 : 8)
+static_assert(DP_LS <= DP_RS, "staged record larger than the HBM record");
+// First DP_LS ints of a record (the remaining slots read as 0).
+__device__ __forceinline__ DpRec dp_ld_lrec(const i32* p) {
+  DpRec r;
+#pragma unroll
+  for (int q = 0; q < DP_RS; ++q) r.v[q] = 0;
+#if DP_LS == 1
+  r.v[0] = p[0];
+#elif DP_LS == 2
+  const int2 t = *(const int2*)p;
+  r.v[0] = t.x; r.v[1] = t.y;
+#elif DP_LS == 4
+  const int4 t = *(const int4*)p;
+  r.v[0] = t.x; r.v[1] = t.y; r.v[2] = t.z; r.v[3] = t.w;
+#else
+  r = dp_ld_rec(p);
+#endif
+  return r;
+}
+__device__ __forceinline__ void dp_st_lrec(i32* p, const DpRec& r) {
+#if DP_LS == 1
+  p[0] = r.v[0];
+#elif DP_LS == 2
+  *(int2*)p = make_int2(r.v[0], r.v[1]);
+#elif DP_LS == 4
+  *(int4*)p = make_int4(r.v[0], r.v[1], r.v[2], r.v[3]);
+#else
+  dp_st_rec(p, r);
+#endif
+}
+
 // Record of the particle in halo cell u at slot `slot`, with its coordinates
 // moved from its cell's centre to the tile's first home cell centre.
 __device__ __forceinline__ DpRec dp_halo_rec(const DpPairArgs& a, const DpTileSmem<DP_D>& S, const DpRep& P,
@@ -284,9 +319,10 @@ __device__ __forceinline__ i64 dp_slot_of(const DpTileSmem<DP_D>& S, int nh, int
   return (i64)S.hstart[u] + (idx - S.hbase[u]);
 }
 
-// Halo records staged per thread and batch (mixed mode).
+// Halo records staged per thread and batch (mixed mode; one for D = 4, where
+// a batch would raise the kernel's register count past 128).
 #ifndef DP_SB
-#define DP_SB 3
+#define DP_SB (DP_D <= 3 ? 3 : 1)
 #endif
 
 // Per-particle pass over the home particles of the staged tile.  LDS selects
@@ -331,7 +367,7 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
     double e = 0.0, w = 0.0, lap = 0.0;
 #if DP_MODE == 1
     // ---------------- mixed: fixed-point displacements ----------------
-    const DpRec ri = LDS ? dp_ld_rec(srec + self * DP_RS) : dp_halo_rec(a, S, P, uh, slot_i);
+    const DpRec ri = LDS ? dp_ld_lrec(srec + self * DP_LS) : dp_halo_rec(a, S, P, uh, slot_i);
     const int ti = (DP_NTYPES > 1) ? ri.v[DP_D] : 0;
     // one chunk of DP_EPL entries at a time, branch-free apart from the rare
     // exact decision near the cut: records first, then independent pair terms
@@ -341,7 +377,7 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
       for (int c = 0; c < DP_EPL; ++c) {
         const int ic = (c < nvalid) ? idx[c] : self;  // padding entries read the particle itself
         if (LDS) {
-          rj[c] = dp_ld_rec(srec + ic * DP_RS);
+          rj[c] = dp_ld_lrec(srec + ic * DP_LS);
         } else {
           const int u = dp_halo_of<DP_D>(S, nh, ic);
           rj[c] = dp_halo_rec(a, S, P, u, (i64)S.hstart[u] + (ic - S.hbase[u]));
@@ -608,7 +644,7 @@ template <int KIND>
 __device__ void dp_tile_run(const DpPairArgs& a) {
   if (!dp_blk_go(a.ctl, a.blk_pos)) return;  // a skipped step of a step block (uniform)
   __shared__ DpTileSmem<DP_D> S;
-  extern __shared__ __align__(16) i32 stage_i[];  // mixed: cap * DP_RS ints; FP64: cap * DP_D doubles + cap types
+  extern __shared__ __align__(16) i32 stage_i[];  // mixed: cap * DP_LS ints; FP64: cap * DP_D doubles + cap types
   const int tid = threadIdx.x;
 #if DP_NTYPES > 1
   __shared__ DpPC spc[DP_NTYPES * DP_NTYPES];
@@ -640,7 +676,7 @@ __device__ void dp_tile_run(const DpPairArgs& a) {
           const int idx = b0 + q * DP_TILE_THREADS;
           if (idx < S.nrec) {
             const int u = dp_halo_of<DP_D>(S, nh, idx);
-            r[q] = dp_ld_rec(a.xq + ((i64)S.hstart[u] + (idx - S.hbase[u])) * DP_RS);
+            r[q] = dp_ld_lrec(a.xq + ((i64)S.hstart[u] + (idx - S.hbase[u])) * DP_RS);
 #pragma unroll
             for (int d = 0; d < DP_D; ++d) sh[q][d] = dp_halo_shift<DP_D>(S, P, u, d);
           }
@@ -651,7 +687,7 @@ __device__ void dp_tile_run(const DpPairArgs& a) {
           if (idx < S.nrec) {
 #pragma unroll
             for (int d = 0; d < DP_D; ++d) r[q].v[d] = (i32)((u32)r[q].v[d] + sh[q][d]);
-            dp_st_rec(stage_i + idx * DP_RS, r[q]);
+            dp_st_lrec(stage_i + idx * DP_LS, r[q]);
           }
         }
       }
