@@ -404,10 +404,12 @@ int refresh_reps(dp_ctx* ctx) {
       if (best < 0) break;
       const i64 nT = prodT / T[best] * (T[best] + 1);
       const i64 nH = prodH / (T[best] + 2) * (T[best] + 3);
-      // stop at a full pass of lane groups, the table limits, or a mean halo
-      // that would not fit the LDS record area (tiles above the mean by more
-      // than the area's slack take the global-memory paths)
-      if (occ * (double)nT > 1.1 * target || nT > DP_HOME_MAX || nH > DP_HALO_MAX ||
+      // stop at a full pass of lane groups (mean occupancy; measured on 2-D
+      // r^-12 at N = 2^22, 9 x 8 home cells of 233 particles mean beat both
+      // 8 x 8 and 9 x 9), the table limits, or a mean halo that would not fit
+      // the LDS record area (tiles above the mean by more than the area's
+      // slack take the global-memory paths)
+      if (occ * (double)nT > (double)target || nT > DP_HOME_MAX || nH > DP_HALO_MAX ||
           occ * (double)nH + 32.0 > (double)lds_rec_max)
         break;
       T[best] += 1;
