@@ -189,16 +189,16 @@ __device__ __forceinline__ DpU4 dp_philox4x32_10(DpU4 c, u32 k0, u32 k1) {
 }
 
 // Two standard normals from 128 random bits (Box-Muller on two 53-bit
-// uniforms strictly inside (0,1)).
+// uniforms strictly inside (0,1)).  The angle 2 pi u2 goes through sincospi
+// (no range reduction: fewer registers than sincos, within an ulp of it).
 __device__ __forceinline__ void dp_box_muller(DpU4 b, double& z0, double& z1) {
   u64 a = ((u64)b.x << 32) | b.y;
   u64 c = ((u64)b.z << 32) | b.w;
   double u1 = ((double)(a >> 11) + 0.5) * 1.1102230246251565404e-16;  // 2^-53
   double u2 = ((double)(c >> 11) + 0.5) * 1.1102230246251565404e-16;
   double rad = sqrt(-2.0 * log(u1));
-  double th = 6.283185307179586476925 * u2;
   double s, co;
-  sincos(th, &s, &co);
+  sincospi(2.0 * u2, &s, &co);
   z0 = rad * co;
   z1 = rad * s;
 }
