@@ -743,6 +743,18 @@ __device__ __forceinline__ void dp_wave_store(double* dst, i64 w0, int nv, const
   __builtin_amdgcn_wave_barrier();
 }
 
+// The DP_D doubles of one particle, two at a time (8-byte aligned pairs:
+// 16-byte loads).
+typedef double dp_d2a8 __attribute__((ext_vector_type(2), aligned(8)));
+__device__ __forceinline__ void dp_ld_vec(const double* p, double (&v)[DP_D]) {
+#pragma unroll
+  for (int d = 0; d + 1 < DP_D; d += 2) {
+    const dp_d2a8 t = *(const dp_d2a8*)(p + d);
+    v[d] = t.x; v[d + 1] = t.y;
+  }
+  if (DP_D & 1) v[DP_D - 1] = p[DP_D - 1];
+}
+
 template <int INTEG>
 __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
   __shared__ double wbuf[4][64 * DP_D];  // one slice per wavefront (256 threads)
@@ -754,14 +766,13 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
   // the particle's state is loaded first, so that the loads are in flight
   // while the replica parameters and the noise are computed
   double x0[DP_D], v0[DP_D], f0[DP_D], b0[DP_D], xr0[DP_D];
+  dp_ld_vec(a.x + ii * DP_D, x0);
+  dp_ld_vec(a.v + ii * DP_D, v0);
+  dp_ld_vec(a.F + ii * DP_D, f0);
 #pragma unroll
-  for (int d = 0; d < DP_D; ++d) {
-    x0[d] = a.x[ii * DP_D + d];
-    v0[d] = a.v[ii * DP_D + d];
-    f0[d] = a.F[ii * DP_D + d];
-    b0[d] = (INTEG == 1) ? a.beta[ii * DP_D + d] : 0.0;
-    xr0[d] = a.wrap ? a.xref[ii * DP_D + d] : 0.0;
-  }
+  for (int d = 0; d < DP_D; ++d) b0[d] = xr0[d] = 0.0;
+  if (INTEG == 1) dp_ld_vec(a.beta + ii * DP_D, b0);
+  if (a.wrap) dp_ld_vec(a.xref + ii * DP_D, xr0);
   const double m = a.mass[ii];
   const u32 pid = (INTEG == 1) ? (u32)a.orig[ii] : 0u;
 #if DP_MODE == 1
