@@ -134,6 +134,31 @@ __global__ void k_skin_check(const double* x, const double* xref, const i32* rep
   dp_raise_rebuild(trip, &ctl->need_rebuild);
 }
 
+// A particle's D doubles, two at a time (8-byte aligned pairs: 16-byte
+// loads and stores, fewer memory instructions per cache line).
+typedef double dp_d2a8 __attribute__((ext_vector_type(2), aligned(8)));
+template <int D>
+__device__ __forceinline__ void dp_ldD(const double* p, double (&v)[D]) {
+#pragma unroll
+  for (int d = 0; d + 1 < D; d += 2) {
+    const dp_d2a8 t = *(const dp_d2a8*)(p + d);
+    v[d] = t.x;
+    v[d + 1] = t.y;
+  }
+  if (D & 1) v[D - 1] = p[D - 1];
+}
+template <int D>
+__device__ __forceinline__ void dp_stD(double* p, const double (&v)[D]) {
+#pragma unroll
+  for (int d = 0; d + 1 < D; d += 2) {
+    dp_d2a8 t;
+    t.x = v[d];
+    t.y = v[d + 1];
+    *(dp_d2a8*)(p + d) = t;
+  }
+  if (D & 1) p[D - 1] = v[D - 1];
+}
+
 // ---------------------------------------------------------------------------
 // Binning: cell of each particle from its wrapped FP64 position; atomic count
 // gives a provisional rank inside the cell.
@@ -151,9 +176,11 @@ __global__ void k_bin(const double* x, const i32* rep, const DpRep* reps, i64 N,
       const DpRep& R = reps[rep[i]];
       int c[D];
       bool finite = true;
+      double xi[D];
+      dp_ldD<D>(x + i * D, xi);
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        double xd = x[i * D + d];
+        double xd = xi[d];
         if (!(xd - xd == 0.0)) { finite = false; xd = 0.0; }
         xd = xd - floor(xd / R.L[d]) * R.L[d];
         int cd = (int)floor(xd * R.inv_w[d]);
@@ -313,13 +340,15 @@ __global__ void k_permute(const i32* perm, i64 N, const double* x, const double*
   i32* sc = so + N;
   DP_FOR(q, N) {
     const i64 p = perm[q];
+    double t[D];
+    dp_ldD<D>(x + p * D, t);
+    dp_stD<D>(sx + q * D, t);
+    dp_ldD<D>(v + p * D, t);
+    dp_stD<D>(sv + q * D, t);
+    dp_ldD<D>(beta + p * D, t);
+    dp_stD<D>(sb + q * D, t);
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      sx[q * D + d] = x[p * D + d];
-      sv[q * D + d] = v[p * D + d];
-      sb[q * D + d] = beta[p * D + d];
-      si[q * D + d] = img[p * D + d];
-    }
+    for (int d = 0; d < D; ++d) si[q * D + d] = img[p * D + d];
     sm[q] = mass[p];
     st[q] = type[p];
     so[q] = orig[p];
@@ -357,16 +386,19 @@ __global__ void k_unpermute_store(i64 N, double* x, double* v, double* beta, dou
     int c[D];
     dp_unlinear<D>((i64)cid - R.c0, R.nc, R.order, c);
     u32 pack = 0;
+    double xs[D], t[D];
+    dp_ldD<D>(sx + q * D, xs);
+    dp_stD<D>(x + q * D, xs);
+    dp_stD<D>(xref + q * D, xs);
+    dp_ldD<D>(sv + q * D, t);
+    dp_stD<D>(v + q * D, t);
+    dp_ldD<D>(sb + q * D, t);
+    dp_stD<D>(beta + q * D, t);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       pack |= ((u32)c[d]) << (DpPack<D>::BITS * d);
-      const double xd = sx[q * D + d];
-      x[q * D + d] = xd;
-      xref[q * D + d] = xd;
-      v[q * D + d] = sv[q * D + d];
-      beta[q * D + d] = sb[q * D + d];
       img[q * D + d] = si[q * D + d];
-      if (xq) xq[q * rs + d] = dp_quantise(xd, ((double)c[d] + 0.5) * R.w[d], R.L[d], R.halfL[d], qinv);
+      if (xq) xq[q * rs + d] = dp_quantise(xs[d], ((double)c[d] + 0.5) * R.w[d], R.L[d], R.halfL[d], qinv);
     }
     mass[q] = sm[q];
     type[q] = st[q];

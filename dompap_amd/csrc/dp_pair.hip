@@ -319,6 +319,18 @@ __device__ __forceinline__ i64 dp_slot_of(const DpTileSmem<DP_D>& S, int nh, int
   return (i64)S.hstart[u] + (idx - S.hbase[u]);
 }
 
+// The DP_D doubles of one particle, two at a time (8-byte aligned pairs:
+// 16-byte loads).
+typedef double dp_d2a8 __attribute__((ext_vector_type(2), aligned(8)));
+__device__ __forceinline__ void dp_ld_vec(const double* p, double (&v)[DP_D]) {
+#pragma unroll
+  for (int d = 0; d + 1 < DP_D; d += 2) {
+    const dp_d2a8 t = *(const dp_d2a8*)(p + d);
+    v[d] = t.x; v[d + 1] = t.y;
+  }
+  if (DP_D & 1) v[DP_D - 1] = p[DP_D - 1];
+}
+
 // Halo records staged per thread and batch (mixed mode; one for D = 4, where
 // a batch would raise the kernel's register count past 128).
 #ifndef DP_SB
@@ -696,8 +708,10 @@ __device__ void dp_tile_run(const DpPairArgs& a) {
       i32* stype = (i32*)(sxd + (size_t)a.cap * DP_D);
       for (int idx = tid; idx < S.nrec; idx += blockDim.x) {
         const i64 slot = dp_slot_of(S, nh, idx);
+        double t[DP_D];
+        dp_ld_vec(a.x + slot * DP_D, t);
 #pragma unroll
-        for (int d = 0; d < DP_D; ++d) sxd[idx * DP_D + d] = a.x[slot * DP_D + d];
+        for (int d = 0; d < DP_D; ++d) sxd[idx * DP_D + d] = t[d];
         if (DP_NTYPES > 1) stype[idx] = a.type[slot];
       }
 #endif
@@ -741,18 +755,6 @@ __device__ __forceinline__ void dp_wave_store(double* dst, i64 w0, int nv, const
     if (e < nv * DP_D) dst[w0 * DP_D + e] = v;
   }
   __builtin_amdgcn_wave_barrier();
-}
-
-// The DP_D doubles of one particle, two at a time (8-byte aligned pairs:
-// 16-byte loads).
-typedef double dp_d2a8 __attribute__((ext_vector_type(2), aligned(8)));
-__device__ __forceinline__ void dp_ld_vec(const double* p, double (&v)[DP_D]) {
-#pragma unroll
-  for (int d = 0; d + 1 < DP_D; d += 2) {
-    const dp_d2a8 t = *(const dp_d2a8*)(p + d);
-    v[d] = t.x; v[d + 1] = t.y;
-  }
-  if (DP_D & 1) v[DP_D - 1] = p[DP_D - 1];
 }
 
 template <int INTEG>
