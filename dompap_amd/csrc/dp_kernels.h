@@ -448,10 +448,12 @@ __device__ __forceinline__ int4 dp_emit_rec(const double* xref, const i32* cellp
                                            const DpTileSmem<D>& S, int u, double qinv) {
   const u32 pack = (u32)cellpack[slot];
   i32 q[4] = {0, 0, 0, 0};
+  double xv[D];
+  dp_ldD<D>(xref + slot * D, xv);
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     const u32 c = (pack >> (DpPack<D>::BITS * d)) & DpPack<D>::MASK;
-    const i32 v = dp_quantise(xref[slot * D + d], ((double)c + 0.5) * P.w[d], P.L[d], P.halfL[d], qinv);
+    const i32 v = dp_quantise(xv[d], ((double)c + 0.5) * P.w[d], P.L[d], P.halfL[d], qinv);
     q[d] = (i32)((u32)v + dp_halo_shift<D>(S, P, u, d));
   }
   return make_int4(q[0], q[1], q[2], q[3]);
