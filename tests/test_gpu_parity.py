@@ -471,3 +471,35 @@ def test_inhomogeneous_tiles_beyond_lds_capacity(precision):
     # a tile inside the occupied half holds about twice the mean occupancy
     dense_halo = 2.0 * x.shape[0] / plan["cells"] * plan["halo_cells_per_tile"]
     assert dense_halo > plan["lds_records"]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_dense_cells_several_particle_groups_per_wavefront(precision):
+    """Uniform random gas at density 6 with a list cutoff of 2: ~48 particles
+    per cell and ~200 neighbours each, so every wavefront of the emit takes
+    its particles of a home cell in more than one group; lists in the
+    oracle's order and forces against the oracle."""
+    from dompap_amd import Simulation
+    rng = np.random.default_rng(41)
+    L = 16.0
+    n = 6 * 16 ** 3
+    x = rng.uniform(0.0, L, size=(n, 3))
+    box = np.array([L] * 3)
+    s = Simulation(seed=5, precision=precision)
+    s.box_vectors = box
+    s.positions = x
+    s.velocities = np.zeros_like(x)
+    s.betas = np.zeros_like(x)
+    s.image_positions = np.zeros(x.shape, dtype=np.int32)
+    s.set_masses(1.0)
+    s.particle_types = np.zeros((n, 1), dtype=np.int32)
+    s.set_pair_potential("(1-r)**2", r_cut=1.0)
+    s.set_neighbor_list(skin=1.0, max_number_of_neighbors=384)
+    F = s.get_forces()
+    nl = np.asarray(s.neighbor_list)
+    nlo, cnt = C.neighbor_list(x, box, 2.0, 384, "cells")
+    assert cnt.max() > 150
+    assert np.array_equal(nl, nlo)
+    Fo = C.pair(x, box, nlo, "harm2", 1.0)["forces"]
+    tol = 1e-10 if precision == "fp64" else 1e-5
+    assert np.abs(F - Fo).max() / _frms(Fo) < tol
