@@ -904,6 +904,60 @@ __device__ __forceinline__ void dp_emit_particles_tab(const DpTileSmem<D>& S, co
   }
 }
 
+// Emit of one tile's rows with one lane per home particle, for cells of a
+// few particles (2-D and 4-D workloads), where the candidate-table pass
+// would leave most lanes idle.  Same rows, in the same order, as
+// dp_emit_particles: each lane walks its stencil cells in order and appends
+// its hits to its own row, four 16-bit (two 32-bit) entries per 8-byte store.
+// Lanes of one home cell read the same staged record at the same time.
+template <int D, typename E>
+__device__ __forceinline__ void dp_emit_lanes(const DpTileSmem<D>& S, const DpRep& P, const double* xref,
+                                              const int4* srq, int K, E* nl, i32* nlcnt, DpCtl* ctl,
+                                              int& st_count) {
+  constexpr int N3 = DpPow3<D>::value;
+  constexpr int EB = 8 * (int)sizeof(E);    // bits per entry
+  constexpr int EPW = 64 / EB;              // entries per 8-byte store
+  const i64 Ks = (i64)((K + 7) & ~7);
+  const int lo16 = P.nlq16_lo, hi16 = P.nlq16_hi;
+  for (int k = threadIdx.x; k < S.nhomep; k += blockDim.x) {
+    const int m = dp_home_of<D>(S, P.nhome, k);
+    const int uh = S.home_u[m];
+    const int loc = k - S.home_p[m];
+    const int self = S.hbase[uh] + loc;
+    const i64 slot = (i64)S.hstart[uh] + loc;
+    const int4 ri = srq[self];
+    const u32 pxy = dp_pack_hi16(ri.x, ri.y), pzw = dp_pack_hi16(ri.z, ri.w);
+    E* row = nl + slot * Ks;
+    u32 cnt = 0;
+    u64 pend = 0;  // entries of the current 8-byte group
+    for (int st = 0; st < N3; ++st) {
+      const int u = uh + S.st_off[st];
+      const int j1 = S.hbase[u + 1];
+      for (int j = S.hbase[u]; j < j1; ++j) {
+        const int4 qj = srq[j];
+        const int s16 = dp_emit_s16<D>(pxy, pzw, dp_pack_hi16(qj.x, qj.y), dp_pack_hi16(qj.z, qj.w));
+        bool h = s16 < lo16 && j != self;
+        // the screen's slack (rare): the exact decision
+        if (!h && s16 <= hi16 && j != self) h = dp_emit_exact<D>(S, P, xref, srq, self, j, slot);
+        if (h) {
+          pend |= (u64)(u32)j << (EB * (cnt % EPW));
+          if (cnt % EPW == EPW - 1) {
+            if (cnt - (EPW - 1) < (u32)K) *(u64*)(row + (cnt - (EPW - 1))) = pend;
+            pend = 0;
+          }
+          ++cnt;
+        }
+      }
+    }
+    // the last, partial group (the entries past cnt are not read)
+    const u32 g0 = cnt - cnt % EPW;
+    if (cnt % EPW != 0 && g0 < (u32)K) *(u64*)(row + g0) = pend;
+    nlcnt[slot] = (i32)cnt;
+    if (cnt > (u32)K) ctl->overflow = 1;
+    st_count = max(st_count, (int)cnt);
+  }
+}
+
 // Tiles whose halo fits the LDS record area; the others are marked in their
 // table (word 6) and emitted by k_emit_tile_far.  Keeping the global-memory
 // path out of this kernel keeps it at <= 128 VGPRs (4 wavefronts per SIMD).
@@ -913,7 +967,8 @@ __global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* 
                                                                const i32* __restrict__ cstart,
                                                                const DpRep* __restrict__ reps, int R,
                                                                i64 ntiles, int K, int cap, E* nl, i32* nlcnt,
-                                                               i32* tiletab, int tstride, DpCtl* ctl, int force) {
+                                                               i32* tiletab, int tstride, DpCtl* ctl, int force,
+                                                               int lanes) {
   if (!force && !ctl->rebuild) return;
   // non-finite positions were binned into arbitrary cells: skip the (then
   // unbounded) emit; the error is reported after the step block
@@ -948,6 +1003,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* 
     const bool cm = P.collapsed_mask != 0;
     if (fits) {
       if (cm) dp_emit_particles<D, E, true, true>(S, P, xref, cellpack, qinv, srq, K, nl, nlcnt, ctl, st_count);
+      else if (lanes) dp_emit_lanes<D, E>(S, P, xref, srq, K, nl, nlcnt, ctl, st_count);
       else dp_emit_particles_tab<D, E>(S, P, xref, srq, ctab, K, nl, nlcnt, ctl, st_count);
     }
     __syncthreads();

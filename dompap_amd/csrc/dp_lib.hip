@@ -192,6 +192,7 @@ struct dp_ctx {
   size_t nl_bytes = 0;
   // tile / entry-format plan (refresh_reps)
   int nlw = 16, ob = 11, tpp = 8, qbits = 24, cap = 1024, cap_emit = 1024;
+  int emit_lanes = 0;  // 1: the emit takes one lane per home particle (cells of a few particles)
   // tuning (0 = automatic)
   int tune_tpp = 0, tune_tile = 0, tune_fused = -1;
   int fused = 0;  // 1: fused force + integrate kernel; 0: force kernel, then integrate kernel
@@ -572,6 +573,13 @@ int refresh_reps(dp_ctx* ctx) {
     ce = std::min<i64>(ce, (i64)(48 * 1024 / 16));
     ctx->cap_emit = (int)std::max<i64>(ce, 64);
   }
+  {
+    // cells of a few particles: one emit lane per particle (the
+    // candidate-table pass takes a wavefront per home cell and group)
+    const int lanes = occ_max < 8.0 ? 1 : 0;
+    if (lanes != ctx->emit_lanes) release_graph(ctx);  // captured emit launches carry the mode
+    ctx->emit_lanes = lanes;
+  }
   // one int4 record and one u16 candidate-table entry per staged particle
   ctx->lds_emit = (size_t)ctx->cap_emit * 16 + ((size_t)ctx->cap_emit * 2 + 15) / 16 * 16;
   ctx->ntiles = t0;
@@ -785,7 +793,7 @@ int launch_emit_te(dp_ctx* ctx, int force) {
   const double qinv = std::ldexp(1.0, ctx->qbits);
   k_emit_tile<D, E><<<ctx->tile_grid, DP_TILE_THREADS, ctx->lds_emit, ctx->st>>>(
       ctx->xref, ctx->cellpack, qinv, ctx->cstart, ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, ctx->cap_emit,
-      (E*)ctx->nl, ctx->nlcnt, ctx->tiletab, ctx->tstride, ctx->ctl_d, force);
+      (E*)ctx->nl, ctx->nlcnt, ctx->tiletab, ctx->tstride, ctx->ctl_d, force, ctx->emit_lanes);
   k_emit_tile_far<D, E><<<ctx->tile_grid, DP_TILE_THREADS, 0, ctx->st>>>(
       ctx->xref, ctx->cellpack, qinv, ctx->reps_d, ctx->ntiles, ctx->K, (E*)ctx->nl, ctx->nlcnt, ctx->tiletab,
       ctx->tstride, ctx->ctl_d, force);
