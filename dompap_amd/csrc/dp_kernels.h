@@ -293,6 +293,130 @@ __global__ void __launch_bounds__(DP_SCAN_THREADS) k_scan_single(i32* count, i64
   if (t == 0) cstart[C] = (i32)N;
 }
 
+// The same scan over many workgroups, for many cells: each workgroup sums
+// its block of 16384 counts (k_scan_blocks), one workgroup scans the block
+// sums (k_scan_top), and each workgroup then writes its block's starts from
+// that offset (k_scan_apply), clearing the counts.
+template <bool APPLY>
+__device__ __forceinline__ void dp_scan_block(i32* count, i64 C, i32* cstart, i32* part, i64 N) {
+  __shared__ i32 wsum[DP_SCAN_THREADS / 64];
+  __shared__ i32 btotal;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int w = t >> 6;
+  constexpr int NW = DP_SCAN_THREADS / 64;
+  const i64 b0 = (i64)blockIdx.x * (DP_SCAN_PER * DP_SCAN_THREADS);
+  const i64 c0 = b0 + DP_SCAN_PER * (i64)t;
+  i32 v[DP_SCAN_PER];
+  if (c0 + DP_SCAN_PER <= C) {
+#pragma unroll
+    for (int k = 0; k < DP_SCAN_PER; k += 4) {
+      const int4 q = *(const int4*)(count + c0 + k);
+      v[k] = q.x; v[k + 1] = q.y; v[k + 2] = q.z; v[k + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < DP_SCAN_PER; ++k) v[k] = (c0 + k < C) ? count[c0 + k] : 0;
+  }
+  i32 sthr = 0;
+#pragma unroll
+  for (int k = 0; k < DP_SCAN_PER; ++k) sthr += v[k];
+  i32 incl = sthr;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const i32 y = __shfl_up(incl, off);
+    if (lane >= off) incl += y;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  if (w == 0) {
+    const i32 xw = (lane < NW) ? wsum[lane] : 0;
+    i32 xi = xw;
+#pragma unroll
+    for (int off = 1; off < NW; off <<= 1) {
+      const i32 y = __shfl_up(xi, off);
+      if (lane >= off) xi += y;
+    }
+    if (lane < NW) wsum[lane] = xi - xw;
+    if (lane == NW - 1) btotal = xi;
+  }
+  __syncthreads();
+  if (!APPLY) {
+    if (t == 0) part[blockIdx.x] = btotal;
+    return;
+  }
+  i32 ex = part[blockIdx.x] + wsum[w] + (incl - sthr);
+  if (c0 + DP_SCAN_PER <= C) {
+#pragma unroll
+    for (int k = 0; k < DP_SCAN_PER; k += 4) {
+      int4 o;
+      o.x = ex; ex += v[k];
+      o.y = ex; ex += v[k + 1];
+      o.z = ex; ex += v[k + 2];
+      o.w = ex; ex += v[k + 3];
+      *(int4*)(cstart + c0 + k) = o;
+      *(int4*)(count + c0 + k) = make_int4(0, 0, 0, 0);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < DP_SCAN_PER; ++k) {
+      if (c0 + k < C) {
+        cstart[c0 + k] = ex;
+        ex += v[k];
+        count[c0 + k] = 0;
+      }
+    }
+  }
+  if (blockIdx.x == 0 && t == 0) cstart[C] = (i32)N;
+}
+__global__ void __launch_bounds__(DP_SCAN_THREADS) k_scan_blocks(i32* count, i64 C, i32* part, const DpCtl* ctl) {
+  if (!ctl->rebuild) return;
+  dp_scan_block<false>(count, C, nullptr, part, 0);
+}
+// Exclusive scan of the nb block sums in place (one workgroup, a running carry).
+__global__ void __launch_bounds__(DP_SCAN_THREADS) k_scan_top(i32* part, i64 nb, const DpCtl* ctl) {
+  if (!ctl->rebuild) return;
+  __shared__ i32 wsum[DP_SCAN_THREADS / 64];
+  __shared__ i32 btotal;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int w = t >> 6;
+  constexpr int NW = DP_SCAN_THREADS / 64;
+  i32 carry = 0;
+  for (i64 b0 = 0; b0 < nb; b0 += DP_SCAN_THREADS) {
+    const i64 b = b0 + t;
+    const i32 x = (b < nb) ? part[b] : 0;
+    i32 incl = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const i32 y = __shfl_up(incl, off);
+      if (lane >= off) incl += y;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    if (w == 0) {
+      const i32 xw = (lane < NW) ? wsum[lane] : 0;
+      i32 xi = xw;
+#pragma unroll
+      for (int off = 1; off < NW; off <<= 1) {
+        const i32 y = __shfl_up(xi, off);
+        if (lane >= off) xi += y;
+      }
+      if (lane < NW) wsum[lane] = xi - xw;
+      if (lane == NW - 1) btotal = xi;
+    }
+    __syncthreads();
+    if (b < nb) part[b] = carry + wsum[w] + (incl - x);
+    carry += btotal;
+    __syncthreads();
+  }
+}
+__global__ void __launch_bounds__(DP_SCAN_THREADS) k_scan_apply(i32* count, i64 C, i32* cstart, const i32* part,
+                                                                 i64 N, const DpCtl* ctl) {
+  if (!ctl->rebuild) return;
+  dp_scan_block<true>(count, C, cstart, const_cast<i32*>(part), N);
+}
+
 // Particle i to position cstart[cell] + rank in the cell-grouped order, with
 // its original id alongside (keys[pos]) for the in-cell sort.
 __global__ void k_scatter(const i32* cell_of, const i32* rank, const i32* cstart, const i32* orig, i64 N, i32* tmp,

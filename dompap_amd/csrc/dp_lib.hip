@@ -205,6 +205,7 @@ struct dp_ctx {
   int pot_even = 0, pair_prec = 64;
   std::string mod_key;
   char* scratch = nullptr;
+  size_t scratch_bytes = 0;
   char* stage = nullptr;
   int cur = 0;
 
@@ -826,7 +827,21 @@ int launch_rebuild_t(dp_ctx* ctx) {
   // the cell counts are zero here: cleared at allocation and by the scan
   k_bin<D><<<gN, DP_BLOCK, 0, st>>>(ctx->x[cur], ctx->rep, ctx->reps_d, N, ctx->cell_of, ctx->rank, ctx->ccount,
                                      ctx->ctl_d);
-  k_scan_single<<<1, DP_SCAN_THREADS, 0, st>>>(ctx->ccount, ctx->Ctot, ctx->cstart, N, ctx->ctl_d);
+  {
+    // cell-count scan: one workgroup for few blocks of 16384 cells, three
+    // passes over many workgroups otherwise (block sums in the scratch
+    // buffer, free until the scatter)
+    const i64 nb = (ctx->Ctot + DP_SCAN_PER * DP_SCAN_THREADS - 1) / (DP_SCAN_PER * DP_SCAN_THREADS);
+    if (nb <= 4 || (size_t)nb * 4 > ctx->scratch_bytes) {
+      k_scan_single<<<1, DP_SCAN_THREADS, 0, st>>>(ctx->ccount, ctx->Ctot, ctx->cstart, N, ctx->ctl_d);
+    } else {
+      i32* part = (i32*)ctx->scratch;
+      k_scan_blocks<<<(unsigned)nb, DP_SCAN_THREADS, 0, st>>>(ctx->ccount, ctx->Ctot, part, ctx->ctl_d);
+      k_scan_top<<<1, DP_SCAN_THREADS, 0, st>>>(part, nb, ctx->ctl_d);
+      k_scan_apply<<<(unsigned)nb, DP_SCAN_THREADS, 0, st>>>(ctx->ccount, ctx->Ctot, ctx->cstart, part, N,
+                                                            ctx->ctl_d);
+    }
+  }
   // the original ids in cell-grouped order go to the head of the scratch
   // buffer, which k_permute overwrites only after the in-cell sort
   i32* keys = (i32*)ctx->scratch;
@@ -1243,7 +1258,8 @@ DP_API int dp_create(dp_ctx** out, int device, int dim, int64_t n_total, int n_r
   CCHK(dalloc(ctx, ctx->tmp, (size_t)N));
   CCHK(dalloc(ctx, ctx->perm, (size_t)N));
   CCHK(dalloc(ctx, ctx->nlcnt, (size_t)N));
-  CCHK(dalloc(ctx, ctx->scratch, (size_t)(N * (3 * D * 8 + 8 + D * 4 + 12))));
+  ctx->scratch_bytes = (size_t)(N * (3 * D * 8 + 8 + D * 4 + 12));
+  CCHK(dalloc(ctx, ctx->scratch, ctx->scratch_bytes));
   CCHK(dalloc(ctx, ctx->stage, (size_t)(N * D * 8)));
   if (hipHostMalloc((void**)&ctx->ctl_h, sizeof(DpCtl), hipHostMallocDefault) != hipSuccess)
     return bail(ctx->fail(DP_ERR_HIP, "hipHostMalloc failed"));
