@@ -1031,7 +1031,7 @@ __device__ __forceinline__ void dp_emit_particles_tab(const DpTileSmem<D>& S, co
 // Emit of one tile's rows with one lane per home particle, for cells of a
 // few particles (2-D and 4-D workloads), where the candidate-table pass
 // would leave most lanes idle.  Same rows, in the same order, as
-// dp_emit_particles: each lane walks its stencil cells in order and appends
+// dp_emit_particles: each lane walks its stencil rows in order and appends
 // its hits to its own row, four 16-bit (two 32-bit) entries per 8-byte store.
 // Lanes of one home cell read the same staged record at the same time.
 template <int D, typename E>
@@ -1054,9 +1054,11 @@ __device__ __forceinline__ void dp_emit_lanes(const DpTileSmem<D>& S, const DpRe
     E* row = nl + slot * Ks;
     u32 cnt = 0;
     u64 pend = 0;  // entries of the current 8-byte group
-    for (int st = 0; st < N3; ++st) {
-      const int u = uh + S.st_off[st];
-      const int j1 = S.hbase[u + 1];
+    // stencil row r = cells 3r .. 3r + 2: consecutive halo cells, one range
+    // of staged records
+    for (int r = 0; r < N3 / 3; ++r) {
+      const int u = uh + S.st_off[3 * r];
+      const int j1 = S.hbase[u + 3];
       for (int j = S.hbase[u]; j < j1; ++j) {
         const int4 qj = srq[j];
         const int s16 = dp_emit_s16<D>(pxy, pzw, dp_pack_hi16(qj.x, qj.y), dp_pack_hi16(qj.z, qj.w));
