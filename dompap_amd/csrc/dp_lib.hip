@@ -822,7 +822,9 @@ template <int D>
 int launch_rebuild_t(dp_ctx* ctx) {
   const int cur = ctx->cur;
   const i64 N = ctx->N;
-  const int gN = grid_for(N);
+  // one thread per particle: these kernels wait on memory, and a capped grid
+  // (a few wavefronts per SIMD, each looping) leaves that latency exposed
+  const int gN = grid_for(N, 1 << 20);
   hipStream_t st = ctx->st;
   // the cell counts are zero here: cleared at allocation and by the scan
   k_bin<D><<<gN, DP_BLOCK, 0, st>>>(ctx->x[cur], ctx->rep, ctx->reps_d, N, ctx->cell_of, ctx->rank, ctx->ccount,
