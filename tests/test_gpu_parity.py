@@ -503,3 +503,35 @@ def test_dense_cells_several_particle_groups_per_wavefront(precision):
     Fo = C.pair(x, box, nlo, "harm2", 1.0)["forces"]
     tol = 1e-10 if precision == "fp64" else 1e-5
     assert np.abs(F - Fo).max() / _frms(Fo) < tol
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_dilute_3d_gas_lane_emit(precision):
+    """3-D LJ gas at density 0.08: ~2 particles per cell, so the emit takes
+    one lane per particle; lists in the oracle's order and forces against
+    the oracle."""
+    from dompap_amd import Simulation
+    rng = np.random.default_rng(43)
+    L = 40.0
+    g = np.arange(17) * (L / 17)
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    x = (x + rng.normal(0, 0.3, size=x.shape)) % L  # jittered sparse lattice: no overlaps
+    n = x.shape[0]
+    box = np.array([L] * 3)
+    s = Simulation(seed=9, precision=precision)
+    s.box_vectors = box
+    s.positions = x
+    s.velocities = np.zeros_like(x)
+    s.betas = np.zeros_like(x)
+    s.image_positions = np.zeros(x.shape, dtype=np.int32)
+    s.set_masses(1.0)
+    s.particle_types = np.zeros((n, 1), dtype=np.int32)
+    s.set_pair_potential("4*(r**-12-r**-6)", r_cut=2.5)
+    s.set_neighbor_list(skin=0.5, max_number_of_neighbors=64)
+    F = s.get_forces()
+    nl = np.asarray(s.neighbor_list)
+    nlo, _ = C.neighbor_list(x, box, 3.0, 64, "cells")
+    assert np.array_equal(nl, nlo)
+    Fo = C.pair(x, box, nlo, "lj", 2.5)["forces"]
+    tol = 1e-10 if precision == "fp64" else 1e-5
+    assert np.abs(F - Fo).max() / _frms(Fo) < tol
