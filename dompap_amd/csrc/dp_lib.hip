@@ -545,8 +545,11 @@ int refresh_reps(dp_ctx* ctx) {
   int nlw = 16;
   if (halo_occ_max * 4.0 + 64.0 >= 65536.0) nlw = 32;
   const int ob = nlw;
-  // LDS records: 1.5x the mean halo, at most 48 KB
-  i64 cap = (i64)std::ceil(1.5 * halo_occ_max) + 64;
+  // LDS records: the mean halo plus three standard deviations of a Poisson
+  // count (at most 1.5x the mean) and 64, at most 48 KB; the rare fuller
+  // tiles take the global-memory path.  (4-D tiles: 2,027 records instead
+  // of 2,817, which brings the force kernel to 4 workgroups per CU.)
+  i64 cap = (i64)std::ceil(std::min(1.5 * halo_occ_max, halo_occ_max + 3.0 * std::sqrt(halo_occ_max))) + 64;
   cap = std::min<i64>(cap, (i64)(48 * 1024 / rec_bytes));
   cap = std::max<i64>(cap, 64);
   // geometry of the built list: cells (a change needs a rebuild) and tiles /
