@@ -26,9 +26,15 @@ Pinning (what checks each function; see tests/test_oracle.py):
   examples/harmonic_repulsive_3d.{csv,toml} (W=5.022787227389184),
   closed-form lattice sums, Newton's third law and the inverse-power-law
   identity W = (n/D) U.
+* ``laplacian``'s first-derivative term (D - 1)(-eps f)/r: central second
+  differences of ``total_energy`` in D = 2, 3, 4.
 * ``make_one_step`` / ``make_one_step_leap_frog``: closed forms of
-  arXiv:1303.7011 Eqs. 16-17 with injected noise and the energy-conservation
-  bound of tests/test_integrator.py:6-104.
+  arXiv:1303.7011 Eqs. 16-17 with injected noise, the energy-conservation
+  bound of tests/test_integrator.py:6-104, and at finite tau three
+  properties the mathematics fixes: the T = 0 terminal velocity F/alpha
+  (the b dt F/m term), the free-particle stationary <m v^2> =
+  T/(1 + dt/2tau) (the b (beta + beta')/2m noise term) and <kappa x^2> = T
+  in a harmonic well.
 * ``wrap_into_box`` / ``minimage``: tests/test_positions.py:12-25,78-93.
 * ``rdf_counts``: closed-form shell counts of simple-cubic and fcc lattices
   (per particle 1 at r = 0, then 6, 12, 8, 6 and 12, 6, 24, 12), the

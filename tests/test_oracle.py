@@ -347,6 +347,175 @@ def test_gjf_closed_forms():
     assert np.array_equal(xc, xo) and np.array_equal(vc, vo) and np.array_equal(bc, bo) and np.array_equal(ic, io)
 
 
+def _gjf_variant(kind):
+    """make_one_step (integrator.py:17-40) with one plausible mistake planted:
+    'b_dropped' (F dt / m without the factor b), 'half_dropped' (b (beta +
+    beta') / m without the 1/2), 'mass_twice' (noise term divided by m^2)."""
+    def step(x, v, beta, F, m, dt, T, tau, xi):
+        alpha = m / tau
+        nb = np.sqrt(2 * alpha * T * dt) * xi
+        num = 1.0 - alpha * dt / 2.0 / m
+        den = 1.0 + alpha * dt / 2.0 / m
+        a, b = num / den, 1 / den
+        fterm = dt * F / m if kind == "b_dropped" else b * dt * F / m
+        if kind == "half_dropped":
+            nterm = b * (beta + nb) / m
+        elif kind == "mass_twice":
+            nterm = b * (beta + nb) / 2 / m / m
+        else:
+            nterm = b * (beta + nb) / 2 / m
+        v = a * v + fterm + nterm
+        return x + v * dt, v, nb
+    return step
+
+
+def _gjf_terminal_velocity(step, m, F, tau, dt, n=3000):
+    x = np.zeros_like(F)
+    v = np.zeros_like(F)
+    b = np.zeros_like(F)
+    z = np.zeros_like(F)
+    for _ in range(n):
+        x, v, b = step(x, v, b, F, m, dt, 0.0, tau, z)
+    return v
+
+
+def test_gjf_terminal_velocity_is_force_over_friction():
+    """T = 0, constant F: the G-JF fixed point v* solves v* = a v* + b dt F / m,
+    i.e. v* (1 - a) = b dt F / m with 1 - a = alpha dt / m * b, so v* = F / alpha
+    (alpha = m / tau) for any dt: pins the b dt F / m term of integrator.py:37."""
+    rng = np.random.default_rng(11)
+    m = rng.uniform(0.5, 3.0, (40, 1))
+    F = rng.normal(size=(40, 3))
+    tau, dt = 0.7, 0.02
+    v = _gjf_terminal_velocity(R.make_one_step, m, F, tau, dt)
+    assert np.allclose(v, F / (m / tau), rtol=1e-10, atol=0)
+    # the C restatement reaches the same fixed point
+    x = np.zeros_like(F)
+    vc, bc, ic = np.zeros_like(F), np.zeros_like(F), np.zeros(F.shape, np.int32)
+    box = np.array([1e9, 1e9, 1e9])
+    for _ in range(3000):
+        x, vc, bc, ic = C.step(x, vc, bc, F, m, dt, 0.0, tau, np.zeros_like(F), box, ic)
+    assert np.allclose(vc, F / (m / tau), rtol=1e-10, atol=0)
+    # planted mistake: without b the fixed point is (1 + alpha dt / 2m) F / alpha
+    vb = _gjf_terminal_velocity(_gjf_variant("b_dropped"), m, F, tau, dt)
+    assert not np.allclose(vb, F / (m / tau), rtol=1e-3, atol=0)
+
+
+def _gjf_mv2(step, seed):
+    """<m v^2> per component of free particles (F = 0) after burn-in."""
+    rng = np.random.default_rng(seed)
+    n = 20000
+    m = np.where(np.arange(n)[:, None] % 2 == 0, 1.0, 2.5)
+    tau, dt, T = 0.05, 0.01, 1.3
+    x = np.zeros((n, 3))
+    v = rng.normal(size=(n, 3)) * np.sqrt(T / m)
+    b = np.zeros((n, 3))
+    F = np.zeros((n, 3))
+    acc, cnt = 0.0, 0
+    for k in range(260):
+        x, v, b = step(x, v, b, F, m, dt, T, tau, rng.normal(size=(n, 3)))
+        if k >= 60:
+            acc += float(np.mean(m * v * v))
+            cnt += 1
+    return acc / cnt, T / (1.0 + dt / (2.0 * tau))
+
+
+def test_gjf_free_particle_kinetic_temperature_closed_form():
+    """F = 0: with c = b / 2m and Var(beta) = 2 alpha T dt, the stationary
+    variance of v_{n+1} = a v_n + c (beta_n + beta_{n+1}) is V = 2 c^2 Var /
+    (1 - a), so <m v^2> = T / (1 + alpha dt / 2m) = T / (1 + dt / 2 tau) per
+    component, for every mass: pins the noise term b (beta + beta') / 2m of
+    integrator.py:37 and the noise amplitude of :29-30 at finite tau."""
+    mv2, expect = _gjf_mv2(R.make_one_step, 5)
+    assert abs(mv2 / expect - 1.0) < 5e-3, (mv2, expect)
+    # planted mistakes: the dropped 1/2 (x4) and a second 1/m (mass-dependent)
+    for kind in ("half_dropped", "mass_twice"):
+        bad, _ = _gjf_mv2(_gjf_variant(kind), 5)
+        assert abs(bad / expect - 1.0) > 0.05, kind
+
+
+def _gjf_harmonic_kx2(step, seed):
+    rng = np.random.default_rng(seed)
+    n = 20000
+    kappa, m, tau, dt, T = 1.0, 1.0, 1.0, 0.05, 0.7
+    x = rng.normal(size=(n, 3)) * np.sqrt(T / kappa)
+    v = rng.normal(size=(n, 3)) * np.sqrt(T / m)
+    b = np.zeros((n, 3))
+    mm = np.full((n, 1), m)
+    acc, cnt = 0.0, 0
+    for k in range(1200):
+        x, v, b = step(x, v, b, -kappa * x, mm, dt, T, tau, rng.normal(size=(n, 3)))
+        if k >= 200:
+            acc += float(np.mean(kappa * x * x))
+            cnt += 1
+    return acc / cnt, T
+
+
+def test_gjf_harmonic_well_configurational_temperature():
+    """Harmonic well F = -kappa x: G-JF samples <kappa x^2> = T exactly at any
+    stable dt (arXiv:1303.7011, the property the method is built for): pins the
+    coupling of the force and noise terms at finite tau."""
+    kx2, T = _gjf_harmonic_kx2(R.make_one_step, 9)
+    assert abs(kx2 / T - 1.0) < 1e-2, kx2
+    bad, _ = _gjf_harmonic_kx2(_gjf_variant("b_dropped"), 9)
+    assert abs(bad / T - 1.0) > 2e-2, bad
+
+
+def _fd_laplacian(x, box, u, i, cut, h=1e-4):
+    """Central second differences of the total energy in particle i's
+    coordinates: sum_d (U(x + h e_id) + U(x - h e_id) - 2 U(x)) / h^2."""
+    def U(y):
+        nl = R.neighbor_list_double_loop(y, box, cut, 64)
+        return R.total_energy(y, box, u, nl, ONE, ONE)
+    u0 = U(x)
+    s = 0.0
+    for d in range(x.shape[1]):
+        xp = x.copy()
+        xm = x.copy()
+        xp[i, d] += h
+        xm[i, d] -= h
+        s += (U(xp) + U(xm) - 2 * u0) / (h * h)
+    return s
+
+
+@pytest.mark.parametrize("dim", [2, 3, 4])
+def test_laplacian_matches_finite_differences(dim):
+    """potential.py:327-364: the per-particle Laplacian eps u'' + (D - 1)
+    (-eps f) / r is the Laplacian of the total energy in that particle's
+    coordinates (sigma = 1, where the reference's missing sigma factors do not
+    matter).  Central differences of total_energy pin the first-derivative
+    term and its (D - 1) factor, which the u''(1) = 12 doctest leaves open
+    (f(1) = 0 there)."""
+    rng = np.random.default_rng(20 + dim)
+    L = 6.0
+    box = np.full(dim, L)
+    # a few particles at pair distances in [0.95, 2.3] (clear of the cut)
+    while True:
+        x = rng.uniform(0, L, (6, dim))
+        ok = True
+        for a in range(6):
+            for b in range(a + 1, 6):
+                r = R.distance(x[a], x[b], box)
+                if r < 0.95 or 2.3 < r < 2.7:
+                    ok = False
+        if ok:
+            break
+    u, f, c = R.make_pair_potential("4*(r**-12-r**-6)", 2.5)
+    nl = R.neighbor_list_double_loop(x, box, 3.0, 64)
+    lap = R.laplacian(x, box, f, c, nl, ONE, ONE)
+    for i in range(6):
+        fd = _fd_laplacian(x, box, u, i, 3.0)
+        assert abs(lap[i] - fd) <= 1e-5 * max(1.0, abs(fd)), (i, lap[i], fd)
+    # planted mistake: D instead of D - 1 in the first-derivative term
+    d1 = np.zeros(6)
+    for i in range(6):
+        for j in nl[i][nl[i] >= 0]:
+            r = R.distance(x[i], x[j], box)
+            d1[i] += -f(r) / r
+    bad = lap + d1
+    assert any(abs(bad[i] - _fd_laplacian(x, box, u, i, 3.0)) > 1e-3 * max(1.0, abs(lap[i])) for i in range(6))
+
+
 # ---------------------------------------------------------------- positions
 def test_positions_kats():
     # tests/test_positions.py:12-93
