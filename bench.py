@@ -175,8 +175,8 @@ def force_errors(F, Fref):
 
 
 def nve_drift(eng, steps, every):
-    """Slope of E_tot/N versus time for a friction-free segment; E_tot =
-    U(x_n) + (K(v_{n-1/2}) + K(v_{n+1/2}))/2 (leap-frog velocities)."""
+    """Slope (per step) of E_tot/N versus step for one friction-free segment;
+    E_tot = U(x_n) + (K(v_{n-1/2}) + K(v_{n+1/2}))/2 (leap-frog velocities)."""
     n = eng.n
     m = eng.get_field("masses")
     ts, es = [], []
@@ -193,6 +193,21 @@ def nve_drift(eng, steps, every):
     e = np.asarray(es)
     slope = float(np.polyfit(t, e, 1)[0])
     return slope, float(e.std())
+
+
+def nve_segments(eng, segments, steps, every, dt):
+    """Drift per particle per unit time over `segments` consecutive
+    friction-free segments of `steps` steps: each segment's own slope, their
+    mean and its standard error (segment-to-segment scatter / sqrt(n))."""
+    slopes, sds = [], []
+    for _ in range(segments):
+        sl, sd = nve_drift(eng, steps, every)
+        slopes.append(sl / dt)
+        sds.append(sd)
+    a = np.asarray(slopes)
+    se = float(a.std(ddof=1) / np.sqrt(len(a))) if len(a) > 1 else None
+    return {"drift_per_particle_per_time": float(a.mean()), "stderr": se, "segments": [float(v) for v in a],
+            "energy_std_per_particle": float(np.mean(sds))}
 
 
 def cpu_baseline(wl, x, rebuild_interval, budget_s=20.0):
@@ -277,7 +292,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-nve", action="store_true", help="skip the NVE drift segment")
-    ap.add_argument("--nve-steps", type=int, default=2000)
+    ap.add_argument("--nve-steps", type=int, default=2000, help="steps per NVE segment")
+    ap.add_argument("--nve-segments", type=int, default=5, help="consecutive NVE segments per precision mode")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy 5 times (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -300,10 +316,15 @@ def main():
         tuning = (args.tpp, args.tile, args.fused)
     eng = make_engine(wl, args.precision, args.pair_precision, tuning)
     eng.step(args.warmup, "gjf")
+    # every step-block graph the timed region replays (events recorded or
+    # not, either buffer parity) is captured and instantiated here, outside
+    # the timed region, whatever --warmup is
+    eng.prepare_step_graphs("gjf", timed=True)
+    eng.prepare_step_graphs("gjf", timed=False)
     sync_all(eng)
     counts = eng.neighbor_counts()
-    kbar = float(counts.mean())
-    log(f"warmup {args.warmup} done, mean list length {kbar:.1f}")
+    kbar0 = float(counts.mean())
+    log(f"warmup {args.warmup} done, mean list length {kbar0:.1f}")
     plan = eng.plan()
     # the tile plan (what the committed traffic figures were measured on)
     tile_plan = {k: v for k, v in plan.items() if k not in ("steps_per_block", "graph_events")}
@@ -311,17 +332,26 @@ def main():
     # per block) with HIP events recorded by the graph around the step kernels
     dt, ms, launches, rebuilds, (f_ms, i_ms) = timed_run(eng, args.steps, timing=True, breakdown=True)
     value = n * args.steps / dt
+    kbar1 = float(eng.neighbor_counts().mean())
+    kbar = 0.5 * (kbar0 + kbar1)  # mean list length over the timed region (its two ends)
     # the same number of steps again, without events
     dtg, _, _, _ = timed_run(eng, args.steps, timing=False)
     nl_ = max(launches, 1)
     log(f"timed {args.steps} steps: {dt:.3f}s -> {value:.4e} particle-steps/s; rebuilds={rebuilds}; "
         f"force kernel {f_ms / nl_ * 1e3:.1f} us, integrate kernel {i_ms / nl_ * 1e3:.1f} us per step")
-    # Algorithmic HBM bytes per particle and launch.  Force kernel: the
-    # particle's neighbour row (entry bytes x mean row length), its staged
-    # record (mixed: RS ints; FP64: D doubles), its row length, its force
-    # (D doubles) and its share of the tile tables.  Integrate kernel: read
-    # x, v, beta, F, xref (5 D doubles), mass, cellpack, orig, rep; write x,
-    # v, beta (3 D doubles); mixed also rewrites its record (RS ints).
+    # Algorithmic HBM bytes per particle and force-kernel launch, as the
+    # measurement contract defines them: B_force = 4 k (one int32 entry per
+    # neighbour, whatever the list's storage) + D s_x (the particle's
+    # coordinates once: 4 bytes each in mixed mode, 8 in FP64) + 8 D (its
+    # force) [+ 4 with several types].
+    typed = "types" in wl
+    s_x = 4 if args.precision == "mixed" else 8
+    b_force = 4.0 * kbar + dim * s_x + 8 * dim + (4 if typed else 0)
+    # The implementation's own bytes: entries of entry_bits, the staged
+    # record (mixed: RS ints; FP64: D doubles), the row length, the force and
+    # the share of the tile tables.  Integrate kernel: read x, v, beta, F,
+    # xref (5 D doubles), mass, cellpack, orig, rep; write x, v, beta (3 D
+    # doubles); mixed also rewrites its record (RS ints).
     ent = plan["entry_bits"] // 8
     rs = 2
     while rs < dim + 1:
@@ -329,7 +359,7 @@ def main():
     home = max(n / max(plan["tiles"], 1), 1.0)
     tab_pp = (8 + 3 * plan["halo_cells_per_tile"] + 2 * plan["home_cells_per_tile"] + 2) * 4 / home
     rec = rs * 4 if args.precision == "mixed" else dim * 8
-    b_force = ent * kbar + rec + 4 + 8 * dim + tab_pp
+    b_force_impl = ent * kbar + rec + 4 + 8 * dim + tab_pp
     b_integ = 8 * dim * 5 + 8 + 4 + 4 + 4 + 8 * dim * 3 + (2 * rs * 4 if args.precision == "mixed" else 0)
     force_s = f_ms / 1e3 / nl_
     integ_s = i_ms / 1e3 / nl_
@@ -358,6 +388,9 @@ def main():
                                                  "dp_tile_force" if not plan["fused"] else "dp_tile_gjf"),
                      "kernel": "dp_tile_force" if not plan["fused"] else "dp_tile_gjf",
                      "bytes_per_particle": b_force, "mean_list_length": kbar,
+                     "bytes_definition": "4*k + D*s_x + 8*D (+4 typed)",
+                     "impl_bytes_per_particle": b_force_impl,
+                     "impl_achieved_GBs": n * b_force_impl / force_s / 1e9,
                      "avg_launch_us": force_s * 1e6,
                      "integrate_kernel": {"kernel": "dp_integrate_gjf", "avg_launch_us": integ_s * 1e6,
                                           "bytes_per_particle": b_integ,
@@ -374,6 +407,7 @@ def main():
         log("FP64 comparison run")
         eng64 = make_engine(wl, "fp64")
         eng64.step(args.warmup, "gjf")
+        eng64.prepare_step_graphs("gjf", timed=True)
         k64 = max(args.steps // 4, 50)
         dt64, ms64, l64, rb64 = timed_run(eng64, k64, timing=True)
         v64 = n * k64 / dt64
@@ -398,10 +432,11 @@ def main():
             e2.set_field("velocities", eng.get_field("velocities"))
             e2.build_neighbor_list()
             e2.set_integrator(wl["dt"], wl["T"], np.inf)
-            slope, sd = nve_drift(e2, args.nve_steps, 100)
-            nve[prec] = {"drift_per_particle_per_time": slope / wl["dt"], "energy_std_per_particle": sd}
+            nve[prec] = nve_segments(e2, args.nve_segments, args.nve_steps, 100, wl["dt"])
             e2.close()
-        out["nve"] = {"steps": args.nve_steps, **nve}
+            log(f"nve {prec}: drift {nve[prec]['drift_per_particle_per_time']:.3e} +- "
+                f"{nve[prec]['stderr'] or 0.0:.1e} per particle per unit time")
+        out["nve"] = {"steps_per_segment": args.nve_steps, "segments": args.nve_segments, **nve}
     if not args.no_cpu:
         log("CPU baseline (C oracle)")
         cb, Fcpu = cpu_baseline(wl, x_now, out["rebuild_interval"])

@@ -1757,22 +1757,24 @@ DP_API int dp_get_neighbor_counts(dp_ctx* ctx, int32_t* out) {
 }
 
 namespace {
+// Largest squared displacement since the last rebuild.  bm_dev: a device
+// buffer of at least grid_for(N) (<= 512) doubles, or null (allocated here).
 template <int D>
-int rdf_max_disp(dp_ctx* ctx, double* disp_sq) {
+int rdf_max_disp(dp_ctx* ctx, double* disp_sq, double* bm_dev) {
   const int g = grid_for(ctx->N);
-  double* bm = nullptr;
-  if (hipMalloc(&bm, (size_t)g * 8) != hipSuccess) return ctx->fail(DP_ERR_HIP, "cannot allocate %d doubles", g);
+  double* bm = bm_dev;
+  if (!bm && hipMalloc(&bm, (size_t)g * 8) != hipSuccess) return ctx->fail(DP_ERR_HIP, "cannot allocate %d doubles", g);
   k_rdf_disp<D><<<g, DP_BLOCK, 0, ctx->st>>>(ctx->x[ctx->cur], ctx->xref, ctx->reps_d, ctx->N, bm);
-  std::vector<double> h((size_t)g);
+  double h[512];  // grid_for caps the grid at 512 workgroups
   int rc = DP_OK;
   if (hipGetLastError() != hipSuccess ||
-      hipMemcpyAsync(h.data(), bm, (size_t)g * 8, hipMemcpyDeviceToHost, ctx->st) != hipSuccess ||
+      hipMemcpyAsync(h, bm, (size_t)g * 8, hipMemcpyDeviceToHost, ctx->st) != hipSuccess ||
       hipStreamSynchronize(ctx->st) != hipSuccess)
     rc = ctx->fail(DP_ERR_HIP, "displacement reduction failed");
-  (void)hipFree(bm);
+  if (!bm_dev) (void)hipFree(bm);
   double m = 0.0;
-  for (double v : h)
-    if (!(v <= m)) m = v;
+  for (int k = 0; k < g && rc == DP_OK; ++k)
+    if (!(h[k] <= m)) m = h[k];
   *disp_sq = m;
   return rc;
 }
@@ -1799,7 +1801,7 @@ double rdf_upper(double e) {
 
 template <int D>
 int launch_rdf_t(dp_ctx* ctx, const double* edges, int ne, unsigned long long* hist_d, const double* thr_d,
-                 double u_last) {
+                 double u_last, double* bm_dev) {
   // the tiles of the last rebuild hold every pair up to the last edge when
   // edges[-1] + 2 * max displacement stays below every cell width
   const DpRep& P = ctx->reps_h[0];
@@ -1807,7 +1809,7 @@ int launch_rdf_t(dp_ctx* ctx, const double* edges, int ne, unsigned long long* h
   for (int d = 0; d < D; ++d) cells = cells && P.nc[d] >= 3;
   if (cells) {
     double dsq = 0.0;
-    CHK(rdf_max_disp<D>(ctx, &dsq));
+    CHK(rdf_max_disp<D>(ctx, &dsq, bm_dev));
     const double reach = edges[ne - 1] + 2.0 * std::sqrt(dsq);
     for (int d = 0; d < D; ++d) cells = cells && reach <= P.w[d] * (1.0 - 1e-9);
   }
@@ -1854,10 +1856,10 @@ DP_API int dp_rdf_counts(dp_ctx* ctx, const double* edges, int n_edges, uint64_t
     rc = ctx->fail(DP_ERR_HIP, "histogram set-up failed");
   if (rc == DP_OK) {
     switch (ctx->D) {
-      case 1: rc = launch_rdf_t<1>(ctx, edges, n_edges, hist_d, edges_d, u_last); break;
-      case 2: rc = launch_rdf_t<2>(ctx, edges, n_edges, hist_d, edges_d, u_last); break;
-      case 3: rc = launch_rdf_t<3>(ctx, edges, n_edges, hist_d, edges_d, u_last); break;
-      case 4: rc = launch_rdf_t<4>(ctx, edges, n_edges, hist_d, edges_d, u_last); break;
+      case 1: rc = launch_rdf_t<1>(ctx, edges, n_edges, hist_d, edges_d, u_last, nullptr); break;
+      case 2: rc = launch_rdf_t<2>(ctx, edges, n_edges, hist_d, edges_d, u_last, nullptr); break;
+      case 3: rc = launch_rdf_t<3>(ctx, edges, n_edges, hist_d, edges_d, u_last, nullptr); break;
+      case 4: rc = launch_rdf_t<4>(ctx, edges, n_edges, hist_d, edges_d, u_last, nullptr); break;
     }
   }
   if (rc == DP_OK && (hipMemcpyAsync(counts, hist_d, (size_t)(n_edges - 1) * 8, hipMemcpyDeviceToHost, ctx->st) !=
@@ -1892,6 +1894,34 @@ DP_API int dp_set_graph_block(dp_ctx* ctx, int steps) {
   if (steps > DP_BLK_MAX) return ctx->fail(DP_ERR_ARG, "step blocks hold at most %d steps", DP_BLK_MAX);
   ctx->graph_block = steps;
   release_graph(ctx);
+  return DP_OK;
+}
+
+DP_API int dp_prepare_step_graphs(dp_ctx* ctx, int integrator, int timed) {
+  if (!ctx) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  if (integrator != DP_INTEGRATOR_GJF && integrator != DP_INTEGRATOR_LEAPFROG)
+    return ctx->fail(DP_ERR_ARG, "unknown integrator %d", integrator);
+  if (!ctx->have_integ) return ctx->fail(DP_ERR_STATE, "dp_set_integrator has not been called");
+  if (ctx->graph_block <= 0) return DP_OK;
+  const int t = timed ? 1 : 0;
+  if (t && !ctx->graph_events_ok) return DP_OK;
+  CHK(prepare(ctx));
+  if (ctx->graph_dirty) {
+    release_graph(ctx);
+    ctx->graph_dirty = false;
+  }
+  for (int par = 0; par < 2; ++par) {
+    if (ctx->gexec[t][par] && ctx->gexec_integ[t][par] == integrator) continue;
+    const int rc = capture_graph(ctx, integrator, par, t);
+    if (rc != DP_OK && t) {
+      // events inside captured graphs unavailable: timed steps launch directly
+      ctx->graph_events_ok = false;
+      (void)hipGetLastError();
+      return DP_OK;
+    }
+    CHK(rc);
+  }
   return DP_OK;
 }
 

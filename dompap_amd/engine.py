@@ -146,6 +146,12 @@ class Engine:
     def set_graph_block(self, steps: int) -> None:
         self._chk(self.lib.dp_set_graph_block(self.ctx, int(steps)))
 
+    def prepare_step_graphs(self, integrator: str = "gjf", timed: bool = False) -> None:
+        """Capture the step-block graphs of both buffer parities now (not at
+        the first step() that replays them)."""
+        integ = _lib.DP_INTEGRATOR_GJF if integrator == "gjf" else _lib.DP_INTEGRATOR_LEAPFROG
+        self._chk(self.lib.dp_prepare_step_graphs(self.ctx, integ, int(bool(timed))))
+
     # -- state ------------------------------------------------------------
     def _field(self, name):
         code, kind, wide = _lib.FIELDS[name]

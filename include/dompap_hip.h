@@ -164,6 +164,13 @@ int dp_get_stats(dp_ctx* ctx, int64_t* nl_updates, int64_t* steps, int32_t* max_
 /* Steps per captured hipGraph block used by dp_step (0 disables graphs). */
 int dp_set_graph_block(dp_ctx* ctx, int steps);
 
+/* Capture (without running) the step-block graphs dp_step replays from either
+ * buffer parity, for `integrator` and the timed (events recorded inside the
+ * graph) or untimed variant, so that a later dp_step spends no time on
+ * capture and instantiation.  No counterpart in the reference (its run loop,
+ * Simulation.py:726-737, has no graphs). */
+int dp_prepare_step_graphs(dp_ctx* ctx, int integrator, int timed);
+
 /* Kernel timing: when enabled, each fused step-kernel launch is bracketed by
  * HIP events on the context stream (steps are launched directly, not from a
  * captured graph, while timing is on); dp_get_kernel_time returns the summed
