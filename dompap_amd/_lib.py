@@ -75,6 +75,7 @@ SIGNATURES = {
     "dp_get_kernel_time": (_c_int, [_vp, _p_dbl, _p_i64]),
     "dp_get_kernel_breakdown": (_c_int, [_vp, _p_dbl, _p_dbl]),
     "dp_rdf_counts": (_c_int, [_vp, _vp, _c_int, _vp]),
+    "dp_rdf_counts_noalloc": (_c_int, [_vp, _vp, _c_int, _vp]),
     "dp_get_rdf_path": (_c_int, [_vp, _p_i32]),
     "dp_compile_check": (_c_int, [_c_int, _c_int, _c_int, _c_int, ctypes.c_char_p, _c_int, ctypes.c_char_p, _c_i64, _p_i64]),
     "dp_compile_check_tuned": (_c_int, [_c_int, _c_int, _c_int, _c_int, ctypes.c_char_p, _c_int, _c_int, _c_int,

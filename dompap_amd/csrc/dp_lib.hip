@@ -21,6 +21,55 @@
 
 #define DP_API extern "C" __attribute__((visibility("default")))
 
+// ---------------------------------------------------------------------------
+// Device scratch of the allocation-free entry point (dp_rdf_counts_noalloc):
+// the bin thresholds, the histogram and the per-workgroup displacement
+// maxima of one call, at most DP_RDF_MAXBINS bins.  Each device holds its own
+// copy until the process ends; calls on one device share it (overlapping
+// calls on one device race).  Host code never passes the symbol itself to a
+// kernel: it uses the device address that hipGetSymbolAddress returns, kept
+// per device.
+// ---------------------------------------------------------------------------
+#define DP_NA_MAX_EDGES (DP_RDF_MAXBINS + 1)
+#define DP_NA_MAX_BLOCKS 512  // grid_for() caps the displacement grid here
+struct DpNoAllocLayout {
+  double thr[DP_NA_MAX_EDGES];
+  unsigned long long hist[DP_RDF_MAXBINS];
+  double block_max[DP_NA_MAX_BLOCKS];
+};
+static __device__ __align__(256) unsigned char scratch[sizeof(DpNoAllocLayout)];
+
+#define DP_MAX_DEVICES 64
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[DP_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device `dev` (looked up once per
+// device, with that device current).
+int scratch_address(int dev, void** out, std::string& err) {
+  if (dev < 0 || dev >= DP_MAX_DEVICES) {
+    err = "device index beyond the scratch table";
+    return DP_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return DP_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return DP_OK;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
+}
+}  // namespace
+
 namespace {
 
 std::string g_create_error;
@@ -221,6 +270,7 @@ struct dp_ctx {
   size_t tiletab_bytes = 0;
   bool nl_valid = true;                 // false when the list buffer was reallocated
   int rdf_path = -1;                    // path of the last dp_rdf_counts: 1 cell tiles, 0 all pairs
+  void* na_scratch = nullptr;           // this device's address of `scratch` (set by dp_create)
   std::vector<i64> cells_sig, tiles_sig;  // geometry the list was built for
   bool pos_dirty = true;
   u64 seed = 0x5eed0d0bull;
@@ -1235,6 +1285,12 @@ DP_API int dp_create(dp_ctx** out, int device, int dim, int64_t n_total, int n_r
   if (hipSetDevice(device) != hipSuccess) return bail(ctx->fail(DP_ERR_HIP, "hipSetDevice failed"));
   if (hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess)
     return bail(ctx->fail(DP_ERR_HIP, "hipStreamCreate failed"));
+  {
+    // device address of the allocation-free entry point's scratch, once per device
+    std::string serr;
+    const int rc = scratch_address(device, &ctx->na_scratch, serr);
+    if (rc != DP_OK) return bail(ctx->fail(rc, "%s", serr.c_str()));
+  }
   const i64 N = n_total, D = dim;
   CCHK(dalloc(ctx, ctx->reps_d, (size_t)n_replicas));
   CCHK(dalloc(ctx, ctx->tab_d, 1));
@@ -1869,6 +1925,45 @@ DP_API int dp_rdf_counts(dp_ctx* ctx, const double* edges, int n_edges, uint64_t
   (void)hipFree(edges_d);
   (void)hipFree(hist_d);
   return rc;
+}
+
+DP_API int dp_rdf_counts_noalloc(dp_ctx* ctx, const double* edges, int n_edges, uint64_t* counts) {
+  if (!ctx || !edges || !counts) return DP_ERR_ARG;
+  (void)hipSetDevice(ctx->dev);
+  if (ctx->R != 1) return ctx->fail(DP_ERR_ARG, "dp_rdf_counts_noalloc works on single-replica contexts");
+  if (n_edges < 2) return ctx->fail(DP_ERR_ARG, "need at least 2 bin edges");
+  if (n_edges > DP_NA_MAX_EDGES)
+    return ctx->fail(DP_ERR_ARG, "dp_rdf_counts_noalloc takes at most %d bin edges (%d bins); got %d",
+                     DP_NA_MAX_EDGES, DP_RDF_MAXBINS, n_edges);
+  for (int k = 1; k < n_edges; ++k)
+    if (!(edges[k] >= edges[k - 1])) return ctx->fail(DP_ERR_ARG, "r_bins array must be sorted in ascending order.");
+  // a new tile plan may allocate: the plan must be current already
+  if (ctx->reps_dirty || ctx->reps_h.empty())
+    return ctx->fail(DP_ERR_STATE,
+                     "the cell plan is not current: build the neighbour list (dp_build_neighbor_list) first");
+  if (!ctx->na_scratch) return ctx->fail(DP_ERR_STATE, "no scratch address for device %d", ctx->dev);
+  DpNoAllocLayout* sc = (DpNoAllocLayout*)ctx->na_scratch;  // device address
+  double thr[DP_NA_MAX_EDGES];
+  for (int k = 0; k < n_edges; ++k) thr[k] = rdf_lower(edges[k]);
+  const double u_last = rdf_upper(edges[n_edges - 1]);
+  const int nb = n_edges - 1;
+  HIPCHK(hipMemcpyAsync(sc->thr, thr, (size_t)n_edges * 8, hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));  // thr lives on this stack frame
+  // the histogram keeps the last call's counts: cleared by a fill kernel
+  k_fill_u64<<<(nb + 255) / 256, 256, 0, ctx->st>>>(sc->hist, nb);
+  HIPCHK(hipGetLastError());
+  int rc = DP_OK;
+  switch (ctx->D) {
+    case 1: rc = launch_rdf_t<1>(ctx, edges, n_edges, sc->hist, sc->thr, u_last, sc->block_max); break;
+    case 2: rc = launch_rdf_t<2>(ctx, edges, n_edges, sc->hist, sc->thr, u_last, sc->block_max); break;
+    case 3: rc = launch_rdf_t<3>(ctx, edges, n_edges, sc->hist, sc->thr, u_last, sc->block_max); break;
+    case 4: rc = launch_rdf_t<4>(ctx, edges, n_edges, sc->hist, sc->thr, u_last, sc->block_max); break;
+    default: rc = ctx->fail(DP_ERR_ARG, "dimension %d unsupported", ctx->D);
+  }
+  if (rc != DP_OK) return rc;
+  HIPCHK(hipMemcpyAsync(counts, sc->hist, (size_t)nb * 8, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return DP_OK;
 }
 
 DP_API int dp_get_rdf_path(dp_ctx* ctx, int* cells) {

@@ -250,6 +250,14 @@ class Engine:
         self._chk(self.lib.dp_rdf_counts(self.ctx, _ptr(e), int(e.size), _ptr(out)))
         return out
 
+    def rdf_counts_noalloc(self, edges) -> np.ndarray:
+        """rdf_counts through the library's allocation-free entry point (at
+        most 4096 bins; the list must have been built)."""
+        e = np.ascontiguousarray(np.asarray(edges, dtype=np.float64))
+        out = np.zeros(max(e.size - 1, 0), dtype=np.uint64)
+        self._chk(self.lib.dp_rdf_counts_noalloc(self.ctx, _ptr(e), int(e.size), _ptr(out)))
+        return out
+
     def rdf_path(self) -> int:
         """1 if the last rdf_counts ran on the cell tiles, 0 for all pairs."""
         v = ctypes.c_int32(-1)

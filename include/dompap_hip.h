@@ -203,6 +203,17 @@ int dp_compile_code(int dim, int precision, int ntypes, int even, const char* de
  * caller normalises by the shell volumes. */
 int dp_rdf_counts(dp_ctx* ctx, const double* edges, int n_edges, uint64_t* counts);
 
+/* dp_rdf_counts without any device or host allocation: no counterpart in the
+ * reference.  For callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The thresholds,
+ * histogram and displacement maxima live in a fixed device array of the
+ * library (one per GPU, ~69 KB, kept until the process ends); at most 4097
+ * edges (4096 bins), else DP_ERR_ARG.  The cell plan must be current (a
+ * neighbour list built since the last change of box or list parameters),
+ * else DP_ERR_STATE.  Calls that overlap on one GPU share the array and
+ * race.  Results equal dp_rdf_counts's. */
+int dp_rdf_counts_noalloc(dp_ctx* ctx, const double* edges, int n_edges, uint64_t* counts);
+
 /* Which kernel the last dp_rdf_counts used: 1 = cell tiles of the current
  * list (last edge + 2 * max displacement < cell width), 0 = all pairs.  No
  * counterpart in the reference (its RDF is always all pairs). */
