@@ -242,6 +242,8 @@ struct dp_ctx {
   // tile / entry-format plan (refresh_reps)
   int nlw = 16, ob = 11, tpp = 8, qbits = 24, cap = 1024, cap_emit = 1024;
   int emit_lanes = 0;  // 1: the emit takes one lane per home particle (cells of a few particles)
+  int nlw_min = 16;    // 32 after a tile's halo outgrew 16-bit entries (kept for later builds)
+  bool last_cell_overflow = false;  // the last check_ctl failed on a halo beyond the entry width
   // tuning (0 = automatic)
   int tune_tpp = 0, tune_tile = 0, tune_fused = -1;
   int fused = 0;  // 1: fused force + integrate kernel; 0: force kernel, then integrate kernel
@@ -592,7 +594,7 @@ int refresh_reps(dp_ctx* ctx) {
   // entry format: the index of a neighbour in its tile's staged halo; 16
   // bits unless a halo could hold more than 2^16 particles
   (void)occ_max;
-  int nlw = 16;
+  int nlw = ctx->nlw_min;
   if (halo_occ_max * 4.0 + 64.0 >= 65536.0) nlw = 32;
   const int ob = nlw;
   // LDS records: the mean halo plus three standard deviations of a Poisson
@@ -988,6 +990,7 @@ int check_ctl(dp_ctx* ctx) {
   HIPCHK(hipMemcpyAsync(ctx->ctl_h, ctx->ctl_d, sizeof(DpCtl), hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   reduce_ctl(ctx->ctl_h);
+  ctx->last_cell_overflow = false;
   if (ctx->ctl_h->overflow || ctx->ctl_h->nonfinite || ctx->ctl_h->cell_overflow) {
     const bool ov = ctx->ctl_h->overflow != 0;
     const bool co = ctx->ctl_h->cell_overflow != 0;
@@ -1003,10 +1006,18 @@ int check_ctl(dp_ctx* ctx) {
                        "Number of neighbors (%d) exceeds max_number_of_neighbors=%d; "
                        "increase max_number_of_neighbors",
                        mc, ctx->K);
-    if (co)
+    if (co) {
+      // the list is incomplete; later builds use 32-bit entries
+      ctx->last_cell_overflow = true;
+      ctx->nlw_min = 32;
+      ctx->reps_dirty = true;
+      ctx->nl_built = false;
+      release_graph(ctx);
       return ctx->fail(DP_ERR_OVERFLOW,
-                       "a cell holds %d particles, more than the %d-bit neighbour entries can address",
-                       ctx->ctl_h->max_cell, ctx->nlw);
+                       "a tile's halo holds %d particles, more than the %d that %d-bit neighbour entries address; "
+                       "the next build uses 32-bit entries",
+                       ctx->ctl_h->max_halo, 1 << 16, ctx->nlw);
+    }
     return ctx->fail(DP_ERR_NONFINITE, "non-finite particle position encountered");
   }
   return DP_OK;
@@ -1554,13 +1565,19 @@ DP_API int dp_get_field(dp_ctx* ctx, int field, void* dst) {
 DP_API int dp_build_neighbor_list(dp_ctx* ctx) {
   if (!ctx) return DP_ERR_ARG;
   (void)hipSetDevice(ctx->dev);
-  CHK(refresh_reps(ctx));
-  k_request_rebuild<<<1, 1, 0, ctx->st>>>(ctx->ctl_d);
-  CHK(launch_prologue(ctx, 0));
-  CHK(launch_rebuild(ctx));
-  ctx->nl_built = true;
-  ctx->pos_dirty = false;
-  return check_ctl(ctx);
+  int rc = DP_OK;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    CHK(refresh_reps(ctx));
+    k_request_rebuild<<<1, 1, 0, ctx->st>>>(ctx->ctl_d);
+    CHK(launch_prologue(ctx, 0));
+    CHK(launch_rebuild(ctx));
+    ctx->nl_built = true;
+    ctx->pos_dirty = false;
+    rc = check_ctl(ctx);
+    // a halo beyond 16-bit entries: rebuilt once with 32-bit entries
+    if (!(rc == DP_ERR_OVERFLOW && ctx->last_cell_overflow)) break;
+  }
+  return rc;
 }
 
 DP_API int dp_update_neighbor_list(dp_ctx* ctx, int* rebuilt) {

@@ -39,7 +39,10 @@ extern "C" {
 #define DP_ERR_NONFINITE (-6) /* a non-finite position was seen */
 
 #define DP_FP64 0   /* pure FP64: matches the FP64 reference to 1e-10 relative */
-#define DP_MIXED 1  /* FP32 pair arithmetic on cell-relative coordinates, FP64 accumulation / state */
+#define DP_MIXED 1  /* int32 fixed-point coordinates relative to the cell centres (exact pair
+                     * displacements); pair terms in FP64 by default (FP32 with pair_precision = 1
+                     * in dp_load_potential); FP64 force accumulation, state, integration and
+                     * energy / virial reductions */
 
 #define DP_FIELD_POSITIONS 0  /* f64 [N][D]  Simulation.positions */
 #define DP_FIELD_VELOCITIES 1 /* f64 [N][D]  Simulation.velocities */
