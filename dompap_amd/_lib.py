@@ -70,6 +70,7 @@ SIGNATURES = {
     "dp_get_neighbor_counts": (_c_int, [_vp, _vp]),
     "dp_get_stats": (_c_int, [_vp, _p_i64, _p_i64, _p_i32]),
     "dp_set_graph_block": (_c_int, [_vp, _c_int]),
+    "dp_set_block_rebuild": (_c_int, [_vp, _c_int]),
     "dp_prepare_step_graphs": (_c_int, [_vp, _c_int, _c_int]),
     "dp_set_kernel_timing": (_c_int, [_vp, _c_int]),
     "dp_get_kernel_time": (_c_int, [_vp, _p_dbl, _p_i64]),

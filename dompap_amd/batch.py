@@ -3,14 +3,15 @@
 The reference runs one ``Simulation`` per process.  ``ReplicaBatch`` takes a
 list of ``Simulation`` objects of one dimension and one pair potential and
 runs them as the replicas of one device context: one launch per kernel for
-all of them, K-step hipGraph blocks, per-replica boxes, cutoffs, skins,
+all of them, 100-step hipGraph blocks, per-replica boxes, cutoffs, skins,
 time steps, temperatures and damping times.  Each replica's noise is keyed by
 its own seed and the particle index inside the replica, so replica r draws
 the numbers a single ``Simulation`` with that seed would draw.
 
-Rebuilds are collective: when any replica's skin trigger fires
-(neighbor_list.py:183-201), every replica rebuilds its list; each replica's
-update counter counts those rebuilds.
+Rebuilds are per replica: before every step the block decides, for each
+replica, whether its own skin trigger fired (neighbor_list.py:183-201) and
+rebuilds only those replicas' lists; each replica's update counter counts its
+own rebuilds, as a single run would.
 """
 from __future__ import annotations
 
@@ -25,7 +26,7 @@ from .potential import device_potential, lower_parameters
 
 class ReplicaBatch:
     def __init__(self, simulations: Sequence, precision: str | None = None, device: int = 0,
-                 seeds: Sequence[int] | None = None, pair_precision: str | None = None):
+                 seeds: Sequence[int] | None = None, pair_precision: str | None = None, block_steps: int = 100):
         sims = list(simulations)
         if not sims:
             raise ValueError("ReplicaBatch needs at least one Simulation")
@@ -75,6 +76,7 @@ class ReplicaBatch:
             raise ValueError("one seed per replica")
         eng.set_seed(seeds[0], int(s0.number_of_steps))
         eng.set_replica_seeds(seeds)
+        eng.set_graph_block(int(block_steps))
         self.seeds = seeds
 
         def cat(name, width, dtype):

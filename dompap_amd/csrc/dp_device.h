@@ -65,7 +65,7 @@ struct DpRep {
 // workgroups serialise, spread ones do not.  The host reduces the slots into
 // max_count / max_cell / max_halo after each read-back.
 #define DP_STAT_SLOTS 64
-#define DP_BLK_MAX 64     // steps per step block, at most
+#define DP_BLK_MAX 128    // steps per step block, at most
 #define DP_STAT_STRIDE 32
 struct DpCtl {
   u64 step_counter;     // steps taken by the device (RNG counter)
@@ -80,10 +80,15 @@ struct DpCtl {
   i32 max_cell;         //   largest cell occupancy at the last rebuild
   i32 max_halo;         //   largest staged halo (records) at the last rebuild
   i32 blk_done;         // steps the last step block completed
-  i32 pad_[4];
+  i32 blk_perstep;      // 1: the block rebuilds (gated) before every step and never stops early
+  i32 pad_[3];
   // step blocks: the rebuild pipeline runs once at the block's start; step p
   // of the block runs while no earlier step of the block raised blk_need and
-  // blk_base + p < blk_target
+  // blk_base + p < blk_target (blk_perstep = 0); or the pipeline runs before
+  // every step and step p runs while blk_base + p < blk_target (blk_perstep =
+  // 1).  Which replicas rebuild is kept per replica outside the block (the
+  // context's flag array: need[R] raised by the skin trigger, then
+  // rebuild[R] for the current step).
   u64 blk_base;         // step counter at the block's start
   u64 blk_target;       // first step index the block must not run
   u64 blk_left;         // steps the host asks for (set before the block)
@@ -153,6 +158,7 @@ struct DpPairArgs {
   DpCtl* ctl;
   i32 wrap;               // 1: wrap positions into the box after the update
   i32 blk_pos;            // position of this step in a step block, -1 outside blocks
+  i32* rflag;             // [2R]: need[r] (raised by replica r's skin trigger), rebuild[r]
 };
 
 #ifndef DP_NO_DEVICE_HELPERS
@@ -293,12 +299,19 @@ __device__ __forceinline__ void dp_raise_rebuild(bool trip, i32* flag) {
     *(volatile i32*)flag = 1;
 }
 
+// Replica r's skin trigger: rare (a few particles per rebuild), so every
+// lane that trips stores the flag itself.
+__device__ __forceinline__ void dp_raise_replica(bool trip, i32* need, int r) {
+  if (trip && need) *(volatile i32*)(need + r) = 1;
+}
+
 // Whether step `pos` of a step block runs (always outside blocks, pos < 0).
 // The earlier steps' flags are read 4 at a time with no early exit, so the
 // loads are independent (scalar loads in kernels that do not write ctl).
 __device__ __forceinline__ bool dp_blk_go(const DpCtl* ctl, int pos) {
   if (pos < 0) return true;
   if (ctl->blk_base + (u64)pos >= ctl->blk_target) return false;
+  if (ctl->blk_perstep) return true;
   const int4* w = (const int4*)ctl->blk_need;
   i32 any = 0;
   for (int q = 0; q < pos; q += 4) {

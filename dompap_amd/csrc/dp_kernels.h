@@ -19,72 +19,109 @@ template <int D> struct DpPack {
 #define DP_FOR(i, n) for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (i64)gridDim.x * blockDim.x)
 
 // ---------------------------------------------------------------------------
-// Control: decide whether this step rebuilds (neighbor_list.py:183-201: the
-// previous integrator pass raised need_rebuild iff some particle's squared
-// displacement exceeds (skin / 2)^2, or is NaN), clear the flag, and take the
-// RNG counter for this step.  Any replica's trigger rebuilds all replicas.
+// Control: decide which replicas rebuild this step (neighbor_list.py:183-201:
+// the previous integrator pass raised need[r] iff some particle of replica r
+// moved by more than half the skin since r's last build, or to a non-finite
+// position), clear the flags, and take the RNG counter for this step.
+// rflag[0..R) = need, rflag[R..2R) = rebuild of the current step; only the
+// replicas whose trigger fired rebuild (all of them on a host request).
 // ---------------------------------------------------------------------------
-__global__ void k_prologue(DpCtl* ctl, const DpRep* reps, int R, i64* nl_updates, int advance_step) {
-  if (blockIdx.x != 0) return;
-  const int tid = threadIdx.x;
-  const int old = ctl->need_rebuild;
-  const int need = old | ctl->force_rebuild;
+// Called by every thread of one workgroup; returns whether any replica
+// rebuilds.  Only skin-triggered rebuilds are counted here; requested ones
+// are counted by the caller that requested them.
+__device__ __forceinline__ int dp_decide_rebuilds(i32* rflag, int R, i64* nl_updates, int forced) {
+  __shared__ int any_s;
+  if (threadIdx.x == 0) any_s = 0;
   __syncthreads();
-  if (need) {
-    for (int k = tid; k < DP_STAT_SLOTS * DP_STAT_STRIDE; k += blockDim.x) ctl->slot[k] = 0;
+  int any = 0;
+  for (int r = threadIdx.x; r < R; r += blockDim.x) {
+    const int nd = rflag[r];
+    const int rb = (nd | forced) ? 1 : 0;
+    rflag[R + r] = rb;
+    rflag[r] = 0;
+    if (nd) nl_updates[r] += 1;
+    any |= rb;
   }
-  if (tid != 0) return;
-  ctl->rebuild = need;
+  if (any) any_s = 1;
+  __syncthreads();
+  return any_s;
+}
+
+__device__ __forceinline__ void dp_clear_stats(DpCtl* ctl) {
+  for (int k = threadIdx.x; k < DP_STAT_SLOTS * DP_STAT_STRIDE; k += blockDim.x) ctl->slot[k] = 0;
+}
+
+__global__ void k_prologue(DpCtl* ctl, i32* rflag, int R, i64* nl_updates, int advance_step) {
+  if (blockIdx.x != 0) return;
+  const int forced = ctl->force_rebuild;
+  const int any = dp_decide_rebuilds(rflag, R, nl_updates, forced);
+  if (any) dp_clear_stats(ctl);
+  if (threadIdx.x != 0) return;
+  ctl->rebuild = any;
   ctl->force_rebuild = 0;
   ctl->need_rebuild = 0;
-  // only skin-triggered rebuilds are counted here; requested ones are counted
-  // by the caller that requested them
-  if (old) {
-    for (int r = 0; r < R; ++r) nl_updates[r] += 1;
-  }
   if (advance_step) {
     ctl->step_index = ctl->step_counter;
     ctl->step_counter += 1;
   }
 }
 
-// Step blocks (a captured graph of `nb` steps with the rebuild pipeline at
-// its start).  k_block_begin takes the place of the per-step prologue for
-// the block's first step: it consumes the rebuild request, clears the
-// block's per-step flags and fixes the block's first step index and the
-// last one it may run (blk_left steps from here, set by the host).
-__global__ void k_block_begin(DpCtl* ctl, i64* nl_updates, int R) {
+// Step blocks (a captured graph of `nb` steps).  k_block_begin clears the
+// block's per-step flags and fixes the block's first step index and the last
+// one it may run (blk_left steps from here, set by the host).  perstep = 0:
+// the rebuild pipeline follows once, at the block's start, and k_block_begin
+// takes the place of the first step's prologue.  perstep = 1: every step
+// starts with k_step_begin and the (gated) pipeline.
+__global__ void k_block_begin(DpCtl* ctl, i64* nl_updates, i32* rflag, int R, int perstep) {
   if (blockIdx.x != 0) return;
   const int tid = threadIdx.x;
-  const int old = ctl->need_rebuild;
-  const int need = old | ctl->force_rebuild;
-  __syncthreads();
-  if (need) {
-    for (int k = tid; k < DP_STAT_SLOTS * DP_STAT_STRIDE; k += blockDim.x) ctl->slot[k] = 0;
-  }
+  const int forced = ctl->force_rebuild;
   for (int k = tid; k < DP_BLK_MAX; k += blockDim.x) ctl->blk_need[k] = 0;
+  int any = 0;
+  if (!perstep) {
+    any = dp_decide_rebuilds(rflag, R, nl_updates, forced);
+    if (any) dp_clear_stats(ctl);
+  }
   if (tid != 0) return;
-  ctl->rebuild = need;
-  ctl->force_rebuild = 0;
-  ctl->need_rebuild = 0;
-  if (old) {
-    for (int r = 0; r < R; ++r) nl_updates[r] += 1;
+  ctl->blk_perstep = perstep;
+  if (!perstep) {
+    ctl->rebuild = any;
+    ctl->force_rebuild = 0;
+    ctl->need_rebuild = 0;
   }
   ctl->blk_base = ctl->step_counter;
   ctl->blk_target = ctl->step_counter + ctl->blk_left;
   ctl->blk_done = 0;
 }
 
-// End of a step block: the steps that ran (up to and including the first
-// one that raised its flag, and below the target), the new step counter and
-// the rebuild request for the next step.
+// Step `pos` of a per-step block: the rebuild decision of that step (none
+// for a step beyond the block's target, whose kernels return at once).
+__global__ void k_step_begin(DpCtl* ctl, i64* nl_updates, i32* rflag, int R, int pos) {
+  if (blockIdx.x != 0) return;
+  if (ctl->blk_base + (u64)pos >= ctl->blk_target) {
+    if (threadIdx.x == 0) ctl->rebuild = 0;
+    return;
+  }
+  const int forced = ctl->force_rebuild;
+  const int any = dp_decide_rebuilds(rflag, R, nl_updates, forced);
+  if (any) dp_clear_stats(ctl);
+  if (threadIdx.x != 0) return;
+  ctl->rebuild = any;
+  ctl->force_rebuild = 0;
+  ctl->need_rebuild = 0;
+}
+
+// End of a step block: the steps that ran (below the target; with perstep =
+// 0 also up to and including the first one that raised its flag), the new
+// step counter and the rebuild request for the next step.
 __global__ void k_block_end(DpCtl* ctl, int nb) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   int done = 0, need = 0;
+  const int perstep = ctl->blk_perstep;
   for (int p = 0; p < nb; ++p) {
     if (ctl->blk_base + (u64)p >= ctl->blk_target) break;
     ++done;
-    if (ctl->blk_need[p]) {
+    if (!perstep && ctl->blk_need[p]) {
       need = 1;
       break;
     }
@@ -93,6 +130,7 @@ __global__ void k_block_end(DpCtl* ctl, int nb) {
   ctl->step_counter = ctl->blk_base + (u64)done;
   if (done > 0) ctl->step_index = ctl->step_counter - 1;
   if (need) ctl->need_rebuild = 1;
+  ctl->blk_perstep = 0;
 }
 
 __global__ void k_block_left(DpCtl* ctl, i64 left) {
@@ -106,8 +144,10 @@ __global__ void k_advance_step(DpCtl* ctl) {
   }
 }
 
-__global__ void k_clear_need(DpCtl* ctl) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->need_rebuild = 0;
+__global__ void k_clear_need(DpCtl* ctl, i32* rflag, int R) {
+  if (blockIdx.x != 0) return;
+  for (int r = threadIdx.x; r < R; r += blockDim.x) rflag[r] = 0;
+  if (threadIdx.x == 0) ctl->need_rebuild = 0;
 }
 
 __global__ void k_request_rebuild(DpCtl* ctl) {
@@ -118,10 +158,11 @@ __global__ void k_request_rebuild(DpCtl* ctl) {
 // after the host uploaded new positions).
 template <int D>
 __global__ void k_skin_check(const double* x, const double* xref, const i32* rep, const DpRep* reps, i64 N,
-                             DpCtl* ctl) {
+                             DpCtl* ctl, i32* need) {
   bool trip = false;
   DP_FOR(i, N) {
-    const DpRep& R = reps[rep[i]];
+    const int r = rep[i];
+    const DpRep& R = reps[r];
     double s = 0.0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -129,7 +170,9 @@ __global__ void k_skin_check(const double* x, const double* xref, const i32* rep
       t = dp_minimage_gt(t, R.L[d], R.halfL[d]);
       s = (d == 0) ? dp_mul(t, t) : dp_add(s, dp_mul(t, t));
     }
-    trip |= dp_skin_trips(s, R.skin_half_sq);
+    const bool ti = dp_skin_trips(s, R.skin_half_sq);
+    dp_raise_replica(ti, need, r);
+    trip |= ti;
   }
   dp_raise_rebuild(trip, &ctl->need_rebuild);
 }
@@ -163,16 +206,22 @@ __device__ __forceinline__ void dp_stD(double* p, const double (&v)[D]) {
 // Binning: cell of each particle from its wrapped FP64 position; atomic count
 // gives a provisional rank inside the cell.
 // ---------------------------------------------------------------------------
+// A replica that does not rebuild this step (rb[r] == 0) keeps its cells:
+// its particles are counted in the cell they were sorted into at its last
+// build (cell_of), so the scan reproduces its cell starts, and the later
+// pipeline kernels leave its particles, records and list untouched.
 template <int D>
 __global__ void k_bin(const double* x, const i32* rep, const DpRep* reps, i64 N, i32* cell_of, i32* rank,
-                      i32* count, DpCtl* ctl) {
+                      i32* count, DpCtl* ctl, const i32* rb) {
   if (!ctl->rebuild) return;
   const int lane = threadIdx.x & 63;
   for (i64 base = (i64)blockIdx.x * blockDim.x; base < N; base += (i64)gridDim.x * blockDim.x) {
     const i64 i = base + threadIdx.x;
     const bool valid = i < N;  // valid lanes form a prefix of the wavefront
     i32 cid = -1;
-    if (valid) {
+    if (valid && !rb[rep[i]]) {
+      cid = cell_of[i];
+    } else if (valid) {
       const DpRep& R = reps[rep[i]];
       int c[D];
       bool finite = true;
@@ -420,9 +469,10 @@ __global__ void __launch_bounds__(DP_SCAN_THREADS) k_scan_apply(i32* count, i64 
 // Particle i to position cstart[cell] + rank in the cell-grouped order, with
 // its original id alongside (keys[pos]) for the in-cell sort.
 __global__ void k_scatter(const i32* cell_of, const i32* rank, const i32* cstart, const i32* orig, i64 N, i32* tmp,
-                          i32* keys, const DpCtl* ctl) {
+                          i32* keys, const DpCtl* ctl, const i32* rep, const i32* rb) {
   if (!ctl->rebuild) return;
   DP_FOR(i, N) {
+    if (!rb[rep[i]]) continue;
     const i32 pos = cstart[cell_of[i]] + rank[i];
     tmp[pos] = (i32)i;
     keys[pos] = orig[i];
@@ -433,9 +483,10 @@ __global__ void k_scatter(const i32* cell_of, const i32* rank, const i32* cstart
 // atomic ranks): the new position of a member is the number of members of
 // its cell with a smaller id, counted over the cell's contiguous keys.
 __global__ void k_cellsort(const i32* tmp, const i32* keys, const i32* cell_of, const i32* cstart, i64 N, i32* perm,
-                           const DpCtl* ctl) {
+                           const DpCtl* ctl, const i32* rep, const i32* rb) {
   if (!ctl->rebuild) return;
   DP_FOR(q, N) {
+    if (!rb[rep[q]]) continue;  // replicas own fixed slot ranges
     const i32 p = tmp[q];
     const i32 c = cell_of[p];
     const i32 lo = cstart[c], hi = cstart[c + 1];
@@ -452,7 +503,7 @@ __global__ void k_cellsort(const i32* tmp, const i32* keys, const i32* cell_of, 
 template <int D>
 __global__ void k_permute(const i32* perm, i64 N, const double* x, const double* v, const double* beta,
                           const double* mass, const i32* img, const i32* type, const i32* orig,
-                          const i32* cell_of, char* scratch, const DpCtl* ctl) {
+                          const i32* cell_of, char* scratch, const DpCtl* ctl, const i32* rep, const i32* rb) {
   if (!ctl->rebuild) return;
   double* sx = (double*)scratch;
   double* sv = sx + N * D;
@@ -463,6 +514,7 @@ __global__ void k_permute(const i32* perm, i64 N, const double* x, const double*
   i32* so = st + N;
   i32* sc = so + N;
   DP_FOR(q, N) {
+    if (!rb[rep[q]]) continue;
     const i64 p = perm[q];
     double t[D];
     dp_ldD<D>(x + p * D, t);
@@ -494,7 +546,7 @@ template <int D>
 __global__ void k_unpermute_store(i64 N, double* x, double* v, double* beta, double* mass, i32* img, i32* type,
                                   i32* orig, i32* cell_of, const char* scratch, double* xref, i32* cellpack,
                                   i32* xq, int rs, double qinv, const i32* rep, const DpRep* reps,
-                                  const DpCtl* ctl) {
+                                  const DpCtl* ctl, const i32* rb) {
   if (!ctl->rebuild) return;
   const double* sx = (const double*)scratch;
   const double* sv = sx + N * D;
@@ -505,6 +557,7 @@ __global__ void k_unpermute_store(i64 N, double* x, double* v, double* beta, dou
   const i32* so = st + N;
   const i32* sc = so + N;
   DP_FOR(q, N) {
+    if (!rb[rep[q]]) continue;
     const DpRep& R = reps[rep[q]];
     const i32 cid = sc[q];
     int c[D];
@@ -1094,7 +1147,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* 
                                                                const DpRep* __restrict__ reps, int R,
                                                                i64 ntiles, int K, int cap, E* nl, i32* nlcnt,
                                                                i32* tiletab, int tstride, DpCtl* ctl, int force,
-                                                               int lanes) {
+                                                               int lanes, const i32* __restrict__ rb) {
   if (!force && !ctl->rebuild) return;
   // non-finite positions were binned into arbitrary cells: skip the (then
   // unbounded) emit; the error is reported after the step block
@@ -1108,6 +1161,8 @@ __global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* 
   const int nwaves = blockDim.x >> 6;
   int st_count = 0, st_cell = 0, st_halo = 0;  // running maxima (per wavefront / thread 0)
   for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    // tiles of replicas that keep their list (uniform over the workgroup)
+    if (!force && !rb[dp_tile_replica(reps, R, t)]) continue;
     dp_tile_setup<D>(S, reps, R, cstart, t, cap);
     const DpRep& P = reps[__builtin_amdgcn_readfirstlane(S.rep)];  // uniform: scalar loads
     const bool fits = S.fits != 0;
@@ -1154,7 +1209,8 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile_far(const double*
                                                                    const i32* __restrict__ cellpack, double qinv,
                                                                    const DpRep* __restrict__ reps, i64 ntiles, int K,
                                                                    E* nl, i32* nlcnt, const i32* tiletab, int tstride,
-                                                                   DpCtl* ctl, int force) {
+                                                                   DpCtl* ctl, int force, int R,
+                                                                   const i32* __restrict__ rb) {
   if (!force && !ctl->rebuild) return;
   if (ctl->nonfinite) return;
   __shared__ DpTileSmem<D> S;
@@ -1162,6 +1218,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile_far(const double*
   for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const i32* tt = tiletab + t * (i64)tstride;
     if (tt[6] == 0) continue;  // uniform over the workgroup
+    if (!force && !rb[dp_tile_replica(reps, R, t)]) continue;
     dp_tile_load<D>(S, tt, 0);
     const DpRep& P = reps[__builtin_amdgcn_readfirstlane(S.rep)];  // uniform: scalar loads
     // st_off is not part of the stored tables

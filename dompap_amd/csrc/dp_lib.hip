@@ -230,6 +230,7 @@ struct dp_ctx {
   DpCtl* ctl_d = nullptr;
   DpCtl* ctl_h = nullptr;  // pinned
   i64* nlupd_d = nullptr;
+  i32* rflag = nullptr;  // [2R]: need[r] (skin trigger of replica r), rebuild[r] (current step)
   double* x[2] = {nullptr, nullptr};
   i32* xq[2] = {nullptr, nullptr};
   double *v = nullptr, *beta = nullptr, *mass = nullptr, *xref = nullptr, *F = nullptr;
@@ -280,6 +281,7 @@ struct dp_ctx {
 
   // graphs
   int graph_block = 16;
+  int block_mode = -1;  // 0: rebuild once per block, 1: before every step (gated per replica), -1: 1 iff R > 1
   hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [timed][starting parity]
   int gexec_integ[2][2] = {{0, 0}, {0, 0}};
   std::vector<DpEv3> blk_ev;            // timed step blocks: events of step p (recorded by the graph)
@@ -826,6 +828,7 @@ DpPairArgs pair_args(dp_ctx* ctx, int cur) {
   a.ctl = ctx->ctl_d;
   a.blk_pos = -1;
   a.wrap = 1;
+  a.rflag = ctx->rflag;
   return a;
 }
 
@@ -852,10 +855,10 @@ int launch_emit_te(dp_ctx* ctx, int force) {
   const double qinv = std::ldexp(1.0, ctx->qbits);
   k_emit_tile<D, E><<<ctx->tile_grid, DP_TILE_THREADS, ctx->lds_emit, ctx->st>>>(
       ctx->xref, ctx->cellpack, qinv, ctx->cstart, ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, ctx->cap_emit,
-      (E*)ctx->nl, ctx->nlcnt, ctx->tiletab, ctx->tstride, ctx->ctl_d, force, ctx->emit_lanes);
+      (E*)ctx->nl, ctx->nlcnt, ctx->tiletab, ctx->tstride, ctx->ctl_d, force, ctx->emit_lanes, ctx->rflag + ctx->R);
   k_emit_tile_far<D, E><<<ctx->tile_grid, DP_TILE_THREADS, 0, ctx->st>>>(
       ctx->xref, ctx->cellpack, qinv, ctx->reps_d, ctx->ntiles, ctx->K, (E*)ctx->nl, ctx->nlcnt, ctx->tiletab,
-      ctx->tstride, ctx->ctl_d, force);
+      ctx->tstride, ctx->ctl_d, force, ctx->R, ctx->rflag + ctx->R);
   HIPCHK(hipGetLastError());
   return DP_OK;
 }
@@ -885,8 +888,9 @@ int launch_rebuild_t(dp_ctx* ctx) {
   const int gN = grid_for(N, 1 << 20);
   hipStream_t st = ctx->st;
   // the cell counts are zero here: cleared at allocation and by the scan
+  const i32* rb = ctx->rflag + ctx->R;  // this step's rebuild flag per replica
   k_bin<D><<<gN, DP_BLOCK, 0, st>>>(ctx->x[cur], ctx->rep, ctx->reps_d, N, ctx->cell_of, ctx->rank, ctx->ccount,
-                                     ctx->ctl_d);
+                                     ctx->ctl_d, rb);
   {
     // cell-count scan: one workgroup for few blocks of 16384 cells, three
     // passes over many workgroups otherwise (block sums in the scratch
@@ -905,14 +909,16 @@ int launch_rebuild_t(dp_ctx* ctx) {
   // the original ids in cell-grouped order go to the head of the scratch
   // buffer, which k_permute overwrites only after the in-cell sort
   i32* keys = (i32*)ctx->scratch;
-  k_scatter<<<gN, DP_BLOCK, 0, st>>>(ctx->cell_of, ctx->rank, ctx->cstart, ctx->orig, N, ctx->tmp, keys, ctx->ctl_d);
-  k_cellsort<<<gN, DP_BLOCK, 0, st>>>(ctx->tmp, keys, ctx->cell_of, ctx->cstart, N, ctx->perm, ctx->ctl_d);
+  k_scatter<<<gN, DP_BLOCK, 0, st>>>(ctx->cell_of, ctx->rank, ctx->cstart, ctx->orig, N, ctx->tmp, keys, ctx->ctl_d,
+                                     ctx->rep, rb);
+  k_cellsort<<<gN, DP_BLOCK, 0, st>>>(ctx->tmp, keys, ctx->cell_of, ctx->cstart, N, ctx->perm, ctx->ctl_d, ctx->rep,
+                                      rb);
   k_permute<D><<<gN, DP_BLOCK, 0, st>>>(ctx->perm, N, ctx->x[cur], ctx->v, ctx->beta, ctx->mass, ctx->img,
-                                         ctx->type, ctx->orig, ctx->cell_of, ctx->scratch, ctx->ctl_d);
+                                         ctx->type, ctx->orig, ctx->cell_of, ctx->scratch, ctx->ctl_d, ctx->rep, rb);
   k_unpermute_store<D><<<gN, DP_BLOCK, 0, st>>>(N, ctx->x[cur], ctx->v, ctx->beta, ctx->mass, ctx->img, ctx->type,
                                                  ctx->orig, ctx->cell_of, ctx->scratch, ctx->xref, ctx->cellpack,
                                                  ctx->xq[cur], ctx->rs, std::ldexp(1.0, ctx->qbits), ctx->rep,
-                                                 ctx->reps_d, ctx->ctl_d);
+                                                 ctx->reps_d, ctx->ctl_d, rb);
   CHK(launch_emit_t<D>(ctx, 0));
   return DP_OK;
 }
@@ -930,9 +936,9 @@ int launch_rebuild(dp_ctx* ctx) {
 template <int D>
 int launch_skin_check_t(dp_ctx* ctx) {
   // the flag of the last integrator pass refers to positions replaced since
-  k_clear_need<<<1, 64, 0, ctx->st>>>(ctx->ctl_d);
+  k_clear_need<<<1, 256, 0, ctx->st>>>(ctx->ctl_d, ctx->rflag, ctx->R);
   k_skin_check<D><<<grid_for(ctx->N), DP_BLOCK, 0, ctx->st>>>(ctx->x[ctx->cur], ctx->xref, ctx->rep, ctx->reps_d,
-                                                              ctx->N, ctx->ctl_d);
+                                                              ctx->N, ctx->ctl_d, ctx->rflag);
   HIPCHK(hipGetLastError());
   return DP_OK;
 }
@@ -980,7 +986,7 @@ void reduce_ctl(DpCtl* c) {
 }
 
 int launch_prologue(dp_ctx* ctx, int advance) {
-  k_prologue<<<1, 256, 0, ctx->st>>>(ctx->ctl_d, ctx->reps_d, ctx->R, ctx->nlupd_d, advance);
+  k_prologue<<<1, 256, 0, ctx->st>>>(ctx->ctl_d, ctx->rflag, ctx->R, ctx->nlupd_d, advance);
   HIPCHK(hipGetLastError());
   return DP_OK;
 }
@@ -1105,6 +1111,14 @@ int resolve_timing(dp_ctx* ctx) {
   return DP_OK;
 }
 
+// Whether step blocks run the (gated) rebuild pipeline before every step,
+// deciding per replica, instead of once at the block's start and stopping at
+// the first skin trigger: the default for several replicas, where some
+// replica's trigger fires on most steps.
+int block_rebuilds_per_step(const dp_ctx* ctx) {
+  return ctx->block_mode >= 0 ? ctx->block_mode : (ctx->R > 1 ? 1 : 0);
+}
+
 // A step block starting from buffer parity `par`: block control, the gated
 // rebuild pipeline once, `graph_block` gated steps, block end.  Steps after
 // the first one whose skin trigger fired are skipped; the host reads how many
@@ -1125,10 +1139,18 @@ int capture_graph(dp_ctx* ctx, int integ, int par, int timed) {
   ctx->cur = par;
   HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
   ctx->capturing = true;
-  k_block_begin<<<1, 256, 0, ctx->st>>>(ctx->ctl_d, ctx->nlupd_d, ctx->R);
-  int rc = launch_rebuild(ctx);
-  for (int s = 0; s < ctx->graph_block && rc == DP_OK; ++s)
+  const int perstep = block_rebuilds_per_step(ctx);
+  k_block_begin<<<1, 256, 0, ctx->st>>>(ctx->ctl_d, ctx->nlupd_d, ctx->rflag, ctx->R, perstep);
+  int rc = perstep ? DP_OK : launch_rebuild(ctx);
+  for (int s = 0; s < ctx->graph_block && rc == DP_OK; ++s) {
+    if (perstep) {
+      // this step's rebuild decision per replica, then the gated pipeline
+      k_step_begin<<<1, 256, 0, ctx->st>>>(ctx->ctl_d, ctx->nlupd_d, ctx->rflag, ctx->R, s);
+      rc = launch_rebuild(ctx);
+      if (rc != DP_OK) break;
+    }
     rc = launch_step_kernels(ctx, integ, nullptr, timed ? &ctx->blk_ev[s] : nullptr, s);
+  }
   k_block_end<<<1, 64, 0, ctx->st>>>(ctx->ctl_d, ctx->graph_block);
   ctx->capturing = false;
   hipError_t e = hipStreamEndCapture(ctx->st, &g);
@@ -1307,6 +1329,7 @@ DP_API int dp_create(dp_ctx** out, int device, int dim, int64_t n_total, int n_r
   CCHK(dalloc(ctx, ctx->tab_d, 1));
   CCHK(dalloc(ctx, ctx->ctl_d, 1));
   CCHK(dalloc(ctx, ctx->nlupd_d, (size_t)n_replicas));
+  CCHK(dalloc(ctx, ctx->rflag, (size_t)2 * n_replicas));
   CCHK(dalloc(ctx, ctx->red, (size_t)n_replicas));
   CCHK(dalloc(ctx, ctx->x[0], (size_t)(N * D)));
   CCHK(dalloc(ctx, ctx->x[1], (size_t)(N * D)));
@@ -1360,6 +1383,8 @@ DP_API int dp_create(dp_ctx** out, int device, int dim, int64_t n_total, int n_r
     ok = ok && hipMemcpy(ctx->ctl_d, &c, sizeof(c), hipMemcpyHostToDevice) == hipSuccess;
     std::vector<u64> z64((size_t)n_replicas, 0);
     ok = ok && hipMemcpy(ctx->nlupd_d, z64.data(), 8 * n_replicas, hipMemcpyHostToDevice) == hipSuccess;
+    std::vector<i32> z32((size_t)2 * n_replicas, 0);
+    ok = ok && hipMemcpy(ctx->rflag, z32.data(), 8 * n_replicas, hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) return bail(ctx->fail(DP_ERR_HIP, "initialising device buffers failed"));
   }
   ctx->offsets = {0, n_total};
@@ -1386,7 +1411,7 @@ DP_API int dp_destroy(dp_ctx* ctx) {
   for (auto& ev : ctx->ev_free)
     for (auto& e : ev.e) (void)hipEventDestroy(e);
   if (ctx->mod) (void)hipModuleUnload(ctx->mod);
-  dfree(ctx->reps_d); dfree(ctx->tab_d); dfree(ctx->ctl_d); dfree(ctx->nlupd_d);
+  dfree(ctx->reps_d); dfree(ctx->tab_d); dfree(ctx->ctl_d); dfree(ctx->nlupd_d); dfree(ctx->rflag);
   dfree(ctx->red); dfree(ctx->x[0]); dfree(ctx->x[1]); dfree(ctx->xq[0]); dfree(ctx->xq[1]);
   dfree(ctx->v); dfree(ctx->beta); dfree(ctx->mass); dfree(ctx->xref); dfree(ctx->F); dfree(ctx->E);
   dfree(ctx->W); dfree(ctx->LAP); dfree(ctx->xi); dfree(ctx->img); dfree(ctx->type); dfree(ctx->orig);
@@ -2006,6 +2031,16 @@ DP_API int dp_set_graph_block(dp_ctx* ctx, int steps) {
   if (steps > DP_BLK_MAX) return ctx->fail(DP_ERR_ARG, "step blocks hold at most %d steps", DP_BLK_MAX);
   ctx->graph_block = steps;
   release_graph(ctx);
+  return DP_OK;
+}
+
+DP_API int dp_set_block_rebuild(dp_ctx* ctx, int mode) {
+  if (!ctx) return DP_ERR_ARG;
+  if (mode < -1 || mode > 1) return ctx->fail(DP_ERR_ARG, "block rebuild mode must be -1, 0 or 1");
+  if (mode != ctx->block_mode) {
+    ctx->block_mode = mode;
+    release_graph(ctx);
+  }
   return DP_OK;
 }
 

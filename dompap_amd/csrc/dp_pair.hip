@@ -717,10 +717,15 @@ __device__ void dp_tile_run(const DpPairArgs& a) {
 #endif
     }
     __syncthreads();
+    bool trip_t = false;  // a particle of this tile (replica S.rep) moved beyond half the skin
     if (fits)
-      dp_tile_particles<KIND, true>(a, S, P, stage_i, spc, trip);
+      dp_tile_particles<KIND, true>(a, S, P, stage_i, spc, trip_t);
     else
-      dp_tile_particles<KIND, false>(a, S, P, stage_i, spc, trip);
+      dp_tile_particles<KIND, false>(a, S, P, stage_i, spc, trip_t);
+    if (KIND == 2 || KIND == 3) {
+      dp_raise_replica(trip_t, a.rflag, S.rep);
+      trip |= trip_t;
+    }
     __syncthreads();
   }
   if (KIND == 2 || KIND == 3) dp_raise_rebuild(trip, dp_need_flag(a.ctl, a.blk_pos));
@@ -856,7 +861,11 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
   dp_wave_store(a.v, w0, nv, v1, buf);
   if (INTEG == 1) dp_wave_store(a.beta, w0, nv, nb, buf);
   dp_wave_store(a.x_out, w0, nv, xn, buf);
-  if (a.wrap) dp_raise_rebuild(valid && dp_skin_trips(sdisp, P.skin_half_sq), dp_need_flag(a.ctl, a.blk_pos));
+  if (a.wrap) {
+    const bool trip = valid && dp_skin_trips(sdisp, P.skin_half_sq);
+    dp_raise_replica(trip, a.rflag, r);
+    dp_raise_rebuild(trip, dp_need_flag(a.ctl, a.blk_pos));
+  }
 }
 
 extern "C" __global__ void __launch_bounds__(256) dp_integrate_gjf(DpPairArgs a) { dp_integrate_body<1>(a); }

@@ -146,6 +146,11 @@ class Engine:
     def set_graph_block(self, steps: int) -> None:
         self._chk(self.lib.dp_set_graph_block(self.ctx, int(steps)))
 
+    def set_block_rebuild(self, mode: int = -1) -> None:
+        """Step blocks rebuild lists once at their start (0) or before every
+        step, per replica (1); -1 chooses 1 for several replicas."""
+        self._chk(self.lib.dp_set_block_rebuild(self.ctx, int(mode)))
+
     def prepare_step_graphs(self, integrator: str = "gjf", timed: bool = False) -> None:
         """Capture the step-block graphs of both buffer parities now (not at
         the first step() that replays them)."""

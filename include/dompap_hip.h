@@ -167,6 +167,15 @@ int dp_get_stats(dp_ctx* ctx, int64_t* nl_updates, int64_t* steps, int32_t* max_
 /* Steps per captured hipGraph block used by dp_step (0 disables graphs). */
 int dp_set_graph_block(dp_ctx* ctx, int steps);
 
+/* How step blocks rebuild lists: 0 = the rebuild pipeline once at the
+ * block's start, the block ending at the first step whose skin trigger fired
+ * (one replica: fewest launches); 1 = the pipeline (gated) before every step,
+ * only the replicas whose own trigger fired rebuilding, blocks never ending
+ * early (many replicas); -1 = 1 iff the context holds several replicas
+ * (default).  Replaces the per-step update_neighbor_list() call of
+ * Simulation.step (Simulation.py:693) inside captured blocks. */
+int dp_set_block_rebuild(dp_ctx* ctx, int mode);
+
 /* Capture (without running) the step-block graphs dp_step replays from either
  * buffer parity, for `integrator` and the timed (events recorded inside the
  * graph) or untimed variant, so that a later dp_step spends no time on

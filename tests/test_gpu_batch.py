@@ -2,8 +2,11 @@
 systems: forces, energies and virials on equal positions, and short G-JF
 trajectories with device noise (replica r keyed like a single run with its
 seed).  FP64 mode: <= 1e-12 relative for static quantities (summation order
-may differ with the tile plan), <= 1e-8 for positions after 25 steps
-(collective rebuilds change neighbour order, hence rounding)."""
+may differ with the tile plan), <= 1e-8 for positions after 25 steps (the
+batch's tiles order the sums differently).  Rebuilds are per replica: each
+replica rebuilds exactly when its single run does.  The two step-block
+modes (one rebuild pipeline per block, or a gated one before every step)
+give bit-identical trajectories."""
 import numpy as np
 import pytest
 
@@ -104,3 +107,38 @@ def test_batch_mixed_many_uneven_replicas():
         F = ref.get_forces()
         assert np.abs(Fb[r] - F).max() / _frms(F) < 1e-5, r
     b.close()
+
+
+def test_batch_rebuilds_per_replica_like_single_runs():
+    from dompap_amd import ReplicaBatch
+    specs = [(5, 0.80, 0.4), (6, 0.90, 2.0), (5, 0.75, 0.8), (4, 1.00, 1.5)]
+    sims = [_replica("fp64", c, rho, T, 300 + i) for i, (c, rho, T) in enumerate(specs)]
+    singles = [s.copy() for s in sims]
+    b = ReplicaBatch(sims, block_steps=100)
+    b.run(150)
+    xb = b.get_field("positions")
+    upd = b.number_of_neighbor_list_updates()
+    counts = []
+    for r, s in enumerate(singles):
+        s.run(150)
+        counts.append(s.number_of_neighbor_list_updates)
+        assert np.abs(xb[r] - np.asarray(s.positions)).max() < 1e-8, r
+    assert list(upd) == counts
+    # hot and cold replicas rebuild at different rates
+    assert len(set(counts)) > 1
+    b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_block_rebuild_modes_give_identical_trajectories(precision):
+    runs = []
+    for mode, block in ((0, 16), (1, 16), (1, 100), (0, 0)):
+        s = _replica(precision, 6, 0.85, 1.2, 77)
+        s.run(1)  # creates the context
+        s._engine.set_block_rebuild(mode)
+        s._engine.set_graph_block(block)
+        s.run(130)
+        runs.append((np.asarray(s.positions).copy(), np.asarray(s.velocities).copy(),
+                     s.number_of_neighbor_list_updates))
+    for x, v, u in runs[1:]:
+        assert np.array_equal(x, runs[0][0]) and np.array_equal(v, runs[0][1]) and u == runs[0][2]
