@@ -110,6 +110,59 @@ def workload(name, seed):
                 dt=0.005, T=T, tau=10.0, K=256, seed=seed)
 
 
+def replica_workload(seed, n_rep=256):
+    """256 independent 3-D LJ systems: N log-uniform in [500, 20000], density
+    uniform in [0.7, 1.0], T uniform in [0.5, 2.0] (seeded); each starts on a
+    seeded subset of an FCC lattice of 4 c^3 >= N sites filling its cubic box,
+    displaced by N(0, 0.05^2), with Gaussian velocities at its T (zero
+    momentum); r_cut 2.5, skin 0.5, dt 0.005, tau 10."""
+    rng = np.random.default_rng(seed)
+    ns = np.rint(np.exp(rng.uniform(math.log(500), math.log(20000), n_rep))).astype(np.int64)
+    rho = rng.uniform(0.7, 1.0, n_rep)
+    T = rng.uniform(0.5, 2.0, n_rep)
+    from dompap_amd.positions import generate_positions
+    ucc = np.array([(0, 0, 0), (.5, .5, 0), (.5, 0, .5), (0, .5, .5)], dtype=np.float64)
+    xs, vs, boxes = [], [], []
+    for r in range(n_rep):
+        n = int(ns[r])
+        c = int(math.ceil((n / 4.0) ** (1.0 / 3.0) - 1e-9))
+        L = (n / rho[r]) ** (1.0 / 3.0)
+        a = L / c
+        sites = generate_positions(ucc, (c, c, c), (a, a, a))
+        x = sites[np.sort(rng.choice(sites.shape[0], size=n, replace=False))]
+        x = (x + rng.normal(0.0, 0.05, x.shape)) % L
+        xs.append(x)
+        vs.append(gaussian_velocities(rng, n, 3, T[r], np.ones(n)))
+        boxes.append(np.full(3, L))
+    x = np.concatenate(xs)
+    return dict(name="replicas-256", desc=f"{n_rep} batched 3D LJ replicas N log-uniform [500,20000] (total "
+                f"{int(ns.sum())}) rho U[0.7,1.0] T U[0.5,2.0] FCC-subset starts rc=2.5 skin=0.5 dt=0.005 G-JF "
+                "tau=10 100-step graph blocks", x=x, v=np.concatenate(vs), m=np.ones(x.shape[0]),
+                box=np.stack(boxes), offsets=np.concatenate([[0], np.cumsum(ns)]).astype(np.int64), T=T,
+                ns=ns, pot="4*(r**-12-r**-6)", r_cut=2.5, skin=0.5, dt=0.005, tau=10.0, K=256, seed=seed)
+
+
+def make_batch_engine(wl, precision, block_steps=100):
+    from dompap_amd.engine import Engine
+    from dompap_amd.potential import device_potential
+    n = wl["x"].shape[0]
+    R = len(wl["ns"])
+    eng = Engine(3, n, R, precision)
+    eng.set_replicas(wl["offsets"], wl["box"])
+    dp = device_potential(wl["pot"], wl["r_cut"])
+    eng.load_potential(dp.code, dp.r_cut, dp.u_rc, [[1.0]], [[1.0]])
+    eng.set_neighbor_params(np.full(R, wl["r_cut"] + wl["skin"]), np.full(R, wl["skin"]), wl["K"])
+    eng.set_integrator(np.full(R, wl["dt"]), wl["T"], np.full(R, wl["tau"]))
+    eng.set_seed(wl["seed"], 0)
+    eng.set_replica_seeds(np.arange(R, dtype=np.uint64) + np.uint64(wl["seed"]))
+    eng.set_graph_block(block_steps)
+    eng.set_field("positions", wl["x"])
+    eng.set_field("velocities", wl["v"])
+    eng.set_field("masses", wl["m"])
+    eng.build_neighbor_list()
+    return eng
+
+
 def make_engine(wl, precision, pair_precision="fp64", tuning=None):
     from dompap_amd.engine import Engine
     from dompap_amd.potential import device_potential
@@ -274,13 +327,130 @@ def measured_traffic(workload_desc, plan, kernel):
     return None
 
 
+def run_replicas(args, log):
+    """The batched-replica workload: one JSON line like main()'s."""
+    wl = replica_workload(args.seed)
+    n = wl["x"].shape[0]
+    R = len(wl["ns"])
+    log(f"workload {wl['desc']} precision={args.precision}")
+    eng = make_batch_engine(wl, args.precision)
+    eng.step(args.warmup, "gjf")
+    eng.prepare_step_graphs("gjf", timed=True)
+    eng.prepare_step_graphs("gjf", timed=False)
+    sync_all(eng)
+    kbar0 = float(eng.neighbor_counts().mean())
+    upd0 = int(eng.stats()[0].sum())
+    eng.set_kernel_timing(True)
+    sync_all(eng)
+    t0 = time.perf_counter()
+    eng.step(args.steps, "gjf")
+    sync_all(eng)
+    dt = time.perf_counter() - t0
+    _, launches = eng.kernel_time()
+    f_ms, i_ms = eng.kernel_breakdown()
+    eng.set_kernel_timing(False)
+    rebuilds = int(eng.stats()[0].sum()) - upd0
+    dtg, _, _, _ = timed_run(eng, args.steps, timing=False)
+    kbar = 0.5 * (kbar0 + float(eng.neighbor_counts().mean()))
+    value = n * args.steps / dt
+    nl_ = max(launches, 1)
+    force_s = f_ms / 1e3 / nl_
+    integ_s = i_ms / 1e3 / nl_
+    s_x = 4 if args.precision == "mixed" else 8
+    b_force = 4.0 * kbar + 3 * s_x + 24
+    achieved = n * b_force / force_s / 1e9
+    per_rep_interval = args.steps * R / max(rebuilds, 1)
+    log(f"timed {args.steps} steps: {dt:.3f}s -> {value:.4e} particle-steps/s; replica rebuilds={rebuilds} "
+        f"(one per {per_rep_interval:.1f} steps and replica); force kernel {force_s * 1e6:.1f} us, "
+        f"integrate kernel {integ_s * 1e6:.1f} us per step")
+    plan = eng.plan()
+    out = {
+        "metric": "particle-steps/s on 1 MI355X, mixed vs FP64 GPU, at stated force error",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "i32+f64" if args.precision == "mixed" else "f64",
+        "data": "synthetic (seeded FCC-subset lattices + Gaussian velocities)",
+        "config": {"workload": wl["desc"] + f" {args.precision}", "n_particles": n, "replicas": R, "dim": 3,
+                   "precision": args.precision, "seed": args.seed},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "dp_tile_force",
+                     "bytes_per_particle": b_force, "bytes_definition": "4*k + D*s_x + 8*D",
+                     "mean_list_length": kbar, "avg_launch_us": force_s * 1e6,
+                     "integrate_kernel": {"kernel": "dp_integrate_gjf", "avg_launch_us": integ_s * 1e6}},
+        "plan": plan,
+        "timing": "HIP events recorded inside the step-block graphs (100-step blocks, per-replica rebuilds)",
+        "graph_replay_value": n * args.steps / dtg,
+        "replica_rebuilds_in_timed_region": rebuilds,
+        "rebuild_interval_per_replica": per_rep_interval,
+    }
+    x_now = eng.get_field("positions")
+    if not args.no_fp64 and args.precision == "mixed":
+        log("FP64 comparison run")
+        e64 = make_batch_engine(wl, "fp64")
+        e64.step(args.warmup, "gjf")
+        e64.prepare_step_graphs("gjf", timed=False)
+        k64 = max(args.steps // 2, 200)
+        dt64, _, _, _ = timed_run(e64, k64, timing=False)
+        v64 = n * k64 / dt64
+        out["fp64_value"] = v64
+        out["mixed_over_fp64"] = value / v64
+        log(f"fp64: {v64:.4e} particle-steps/s (ratio {value / v64:.2f})")
+        e64.set_field("positions", x_now)
+        r64 = e64.compute(forces=True, energy=True, virial=True)
+        rmx = eng.compute(forces=True, energy=True, virial=True)
+        out["mixed_vs_fp64_gpu"] = force_errors(rmx["forces"], r64["forces"])
+        out["mixed_vs_fp64_gpu"]["energy_rel_max"] = float(np.max(np.abs(rmx["energy"] - r64["energy"]) /
+                                                                  np.abs(r64["energy"])))
+        e64.close()
+    if not args.no_cpu:
+        log("CPU baseline (C oracle)")
+        from oracle import c_oracle as C
+        cores = C.threads()
+        # a bounded sample: the first replicas up to ~100,000 particles, one
+        # list build each (amortised over the measured rebuild interval) and
+        # three force + G-JF steps each
+        t_nl = t_st = 0.0
+        n_s = 0
+        for r in range(R):
+            lo, hi = int(wl["offsets"][r]), int(wl["offsets"][r + 1])
+            x = x_now[lo:hi]
+            box = wl["box"][r]
+            t0 = time.perf_counter()
+            nl, _ = C.neighbor_list(x, box, wl["r_cut"] + wl["skin"], wl["K"], "cells")
+            t_nl += time.perf_counter() - t0
+            v = wl["v"][lo:hi].copy()
+            beta = np.zeros_like(v)
+            img = np.zeros(x.shape, np.int32)
+            t0 = time.perf_counter()
+            for k in range(3):
+                F = C.pair(x, box, nl, "lj", wl["r_cut"])["forces"]
+                xi = C.normals(hi - lo, 3, 1, k)
+                x, v, beta, img = C.step(x, v, beta, F, wl["m"][lo:hi], wl["dt"], float(wl["T"][r]), wl["tau"], xi,
+                                         box, img)
+            t_st += (time.perf_counter() - t0) / 3
+            n_s += hi - lo
+            if n_s >= 100000:
+                break
+        per_step = t_st + t_nl / max(per_rep_interval, 1.0)
+        out["cpu_baseline"] = {"value": n_s / per_step, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+                               "sample": f"C oracle FP64 (OpenMP) on the first {r + 1} replicas ({n_s} particles): "
+                                         f"one cell-list build each ({t_nl:.2f}s in all) + 3 force+G-JF steps each "
+                                         f"({t_st:.3f}s per step in all), rebuild every {per_rep_interval:.1f} "
+                                         "steps per replica as measured on the GPU"}
+        log(f"cpu: {out['cpu_baseline']['value']:.3e} particle-steps/s on {cores} threads")
+    else:
+        out["cpu_baseline"] = None
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", default="lj-1m",
-                    choices=["lj-1m", "lj-4k", "lj-binary-262k", "harmonic-4d-2m", "ipl-2d-4m"])
+                    choices=["lj-1m", "lj-4k", "lj-binary-262k", "harmonic-4d-2m", "ipl-2d-4m", "replicas-256"])
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--pair-precision", default="fp64", choices=["fp64", "fp32"],
                     help="mixed mode: precision of the in-range pair terms")
@@ -308,6 +478,9 @@ def main():
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
+    if args.workload == "replicas-256":
+        run_replicas(args, log)
+        return
     wl = workload(args.workload, args.seed)
     n, dim = wl["x"].shape
     log(f"workload {wl['desc']} precision={args.precision}")

@@ -55,6 +55,8 @@ SIGNATURES = {
     "dp_set_integrator": (_c_int, [_vp, _vp, _vp, _vp]),
     "dp_set_neighbor_params": (_c_int, [_vp, _vp, _vp, _c_int, _c_int]),
     "dp_load_potential": (_c_int, [_vp, ctypes.c_char_p, _c_int, _c_dbl, _c_dbl, _c_int, _vp, _vp, _c_int]),
+    "dp_load_potentials": (_c_int, [_vp, _c_int, ctypes.POINTER(ctypes.c_char_p), _vp, _vp, _vp, _c_int, _vp, _vp,
+                                    _c_int, _vp]),
     "dp_set_field": (_c_int, [_vp, _c_int, _vp]),
     "dp_get_field": (_c_int, [_vp, _c_int, _vp]),
     "dp_build_neighbor_list": (_c_int, [_vp]),
@@ -81,6 +83,8 @@ SIGNATURES = {
     "dp_compile_check": (_c_int, [_c_int, _c_int, _c_int, _c_int, ctypes.c_char_p, _c_int, ctypes.c_char_p, _c_i64, _p_i64]),
     "dp_compile_check_tuned": (_c_int, [_c_int, _c_int, _c_int, _c_int, ctypes.c_char_p, _c_int, _c_int, _c_int,
                                         _c_int, ctypes.c_char_p, _c_i64, _p_i64]),
+    "dp_compile_check_potentials": (_c_int, [_c_int, _c_int, _c_int, _c_int, ctypes.POINTER(ctypes.c_char_p), _vp,
+                                             _c_int, ctypes.c_char_p, _c_i64, _p_i64]),
     "dp_compile_code": (_c_int, [_c_int, _c_int, _c_int, _c_int, ctypes.c_char_p, _c_int, ctypes.c_char_p, _c_i64, _p_i64]),
     "dp_sync": (_c_int, [_vp]),
     "dp_get_stream": (_c_int, [_vp, ctypes.POINTER(_vp)]),

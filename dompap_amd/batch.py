@@ -1,8 +1,9 @@
 """Batched replicas: many independent systems stepped together on one GPU.
 
 The reference runs one ``Simulation`` per process.  ``ReplicaBatch`` takes a
-list of ``Simulation`` objects of one dimension and one pair potential and
-runs them as the replicas of one device context: one launch per kernel for
+list of ``Simulation`` objects of one dimension (and up to 8 distinct pair
+potentials / parameter tables, each replica computing with its own) and runs
+them as the replicas of one device context: one launch per kernel for
 all of them, 100-step hipGraph blocks, per-replica boxes, cutoffs, skins,
 time steps, temperatures and damping times.  Each replica's noise is keyed by
 its own seed and the particle index inside the replica, so replica r draws
@@ -33,22 +34,36 @@ class ReplicaBatch:
         self.simulations = sims
         s0 = sims[0]
         dim = s0.get_dimensions_of_space()
-        pot = (s0.pair_potential_str, float(s0.pair_potential_r_cut))
         tabs = []
         for s in sims:
             if s.get_dimensions_of_space() != dim:
                 raise ValueError("all replicas must have the same dimension of space")
-            if (s.pair_potential_str, float(s.pair_potential_r_cut)) != pot:
-                raise ValueError("all replicas must use the same pair potential and r_cut")
+            if s.pair_potential_str is None:
+                raise ValueError("every replica needs a pair potential (set_pair_potential)")
             types = np.asarray(s.particle_types).reshape(-1)
             if types.shape[0] != s.number_of_particles():
                 types = np.zeros(s.number_of_particles(), dtype=np.int32)
             tabs.append(lower_parameters(s.sigma_func, s.epsilon_func, types,
                                          s.__dict__.get("_sigma_const"), s.__dict__.get("_epsilon_const")))
-        for t in tabs[1:]:
-            if t.sigma.shape != tabs[0].sigma.shape or not (np.array_equal(t.sigma, tabs[0].sigma)
-                                                            and np.array_equal(t.epsilon, tabs[0].epsilon)):
-                raise ValueError("all replicas must have the same sigma / epsilon type tables")
+        # one device potential per distinct (expression, r_cut, sigma / epsilon
+        # tables); the type tables share one size (padded with unused types)
+        nt = max(t.sigma.shape[0] for t in tabs)
+
+        def pad(a):
+            out = np.ones((nt, nt))
+            out[: a.shape[0], : a.shape[1]] = a
+            return out
+
+        keys, rep_pot = [], []
+        for s, t in zip(sims, tabs):
+            key = (s.pair_potential_str, float(s.pair_potential_r_cut), pad(t.sigma).tobytes(),
+                   pad(t.epsilon).tobytes())
+            if key not in keys:
+                keys.append(key)
+            rep_pot.append(keys.index(key))
+        if len(keys) > 8:
+            raise ValueError(f"a batch holds at most 8 distinct potentials / parameter tables, got {len(keys)}")
+        self.rep_pot = np.asarray(rep_pot, dtype=np.int32)
         self.dim = dim
         self.counts = np.array([s.number_of_particles() for s in sims], dtype=np.int64)
         self.offsets = np.concatenate([[0], np.cumsum(self.counts)]).astype(np.int64)
@@ -60,8 +75,11 @@ class ReplicaBatch:
         self.engine = eng
         boxes = np.stack([np.asarray(s.box_vectors, dtype=np.float64).reshape(dim) for s in sims])
         eng.set_replicas(self.offsets, boxes)
-        dp = device_potential(*pot)
-        eng.load_potential(dp.code, dp.r_cut, dp.u_rc, tabs[0].sigma, tabs[0].epsilon, pp)
+        dps = [device_potential(k[0], k[1]) for k in keys]
+        first = [rep_pot.index(p) for p in range(len(keys))]
+        eng.load_potentials([d.code for d in dps], [d.r_cut for d in dps], [d.u_rc for d in dps],
+                            [pad(tabs[i].sigma) for i in first], [pad(tabs[i].epsilon) for i in first],
+                            self.rep_pot, pp)
         # Simulation.py:353-356 per replica
         cut = np.array([t.sigma_max * float(s.pair_potential_r_cut) + float(s.neighbor_list_skin)
                         for s, t in zip(sims, tabs)])

@@ -22,6 +22,7 @@ typedef uint16_t u16;
 
 #define DP_DMAX 4          // largest spatial dimension supported (stencil rows fit one wavefront)
 #define DP_MAXTYPES 8      // largest number of particle types (sigma/eps tables)
+#define DP_MAXPOT 8        // largest number of pair potentials in one context (one per replica)
 #define DP_HALO_MAX 512    // halo cells per tile
 #define DP_HOME_MAX 128    // home cells per tile
 #define DP_TILE_THREADS 256
@@ -57,6 +58,7 @@ struct DpRep {
   i32 nhome;            // prod(T)
   i32 collapsed_mask;   // bit d set if nc[d] == 1
   i32 order;            // 0: dimension 0 fastest, 1: dimension D-1 fastest
+  i32 pot;              // potential of the replica (index into the context's pair tables)
 };
 
 // Device control block (one per context).  The neighbour-list statistics are
@@ -270,13 +272,24 @@ __device__ __forceinline__ double dp_minimage_lt(double d, double L, double half
 template <int D> struct DpPow3 { static constexpr int value = 3 * DpPow3<D - 1>::value; };
 template <> struct DpPow3<0> { static constexpr int value = 1; };
 
+// Capacity of the tile tables in LDS: DP_HALO_MAX / DP_HOME_MAX cells, or, in
+// the run-time compiled pair kernels, the plan's largest halo / home cell
+// counts (DP_TS_HALO / DP_TS_HOME, defined by the host): 3-D tiles have ~48
+// halo cells, and the 7 KB of a 512-cell table would cost a workgroup per CU
+// when the staged records are large.
+#ifndef DP_TS_HALO
+#define DP_TS_HALO DP_HALO_MAX
+#endif
+#ifndef DP_TS_HOME
+#define DP_TS_HOME DP_HOME_MAX
+#endif
 template <int D>
 struct DpTileSmem {
-  i32 hstart[DP_HALO_MAX];     // first slot of each halo cell
-  i32 hcoord[DP_HALO_MAX];     // halo-box coordinates of each halo cell, 8 bits per dimension
-  i32 hbase[DP_HALO_MAX + 1];  // staged-record offset of each halo cell (prefix)
-  i32 home_u[DP_HOME_MAX];     // halo index of each home cell (-1: outside the box)
-  i32 home_p[DP_HOME_MAX + 1]; // home-particle prefix over home cells
+  i32 hstart[DP_TS_HALO];      // first slot of each halo cell
+  i32 hcoord[DP_TS_HALO];      // halo-box coordinates of each halo cell, 8 bits per dimension
+  i32 hbase[DP_TS_HALO + 1];   // staged-record offset of each halo cell (prefix)
+  i32 home_u[DP_TS_HOME];      // halo index of each home cell (-1: outside the box)
+  i32 home_p[DP_TS_HOME + 1];  // home-particle prefix over home cells
   i32 st_off[DpPow3<D>::value];// halo-index offset of each stencil entry
   i32 nrec;                    // staged records in the halo
   i32 maxc;                    // largest halo-cell occupancy

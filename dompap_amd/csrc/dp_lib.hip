@@ -131,6 +131,7 @@ CodeCache& code_cache() {
 struct PairConfig {
   int D, mode, ntypes, even, rs, tpp, nlw, ob, pairprec;
   int cmask;  // dimensions with a single cell in some replica
+  int hmax = 0, omax = 0;  // LDS tile-table capacity: largest halo / home cell count of the plan (0: the limits)
 };
 
 // Stencil-index bits of a neighbour entry (3^D stencil cells).
@@ -142,19 +143,55 @@ int stencil_bits(int D) {
   return b;
 }
 
-std::string pair_source(const PairConfig& c, const char* device_src) {
+// Potential-dependent section of the pair-kernel source (between the
+// markers), compiled once per potential inside namespace dp_p<k>.
+void split_pair_source(std::string& pre, std::string& pot, std::string& post) {
+  const std::string all = dp_src_pair;
+  const std::string b = "//@@POTENTIAL_BEGIN", e = "//@@POTENTIAL_END";
+  const size_t i = all.find(b), j = all.find(e);
+  pre = all.substr(0, i);
+  pot = all.substr(i, j - i);
+  post = all.substr(j);
+}
+
+// The pair kernels' source for potentials srcs[k] (generated device
+// functions; evens[k] = 1 when they take s = x^2), replica r using potential
+// DpRep::pot.
+std::string pair_source(const PairConfig& c, const std::vector<std::string>& srcs, const std::vector<int>& evens) {
   std::string src;
   char buf[512];
   snprintf(buf, sizeof(buf),
-           "#define DP_D %d\n#define DP_MODE %d\n#define DP_NTYPES %d\n#define DP_EVEN %d\n#define DP_RS %d\n"
+           "#define DP_D %d\n#define DP_MODE %d\n#define DP_NTYPES %d\n#define DP_NPOT %d\n#define DP_RS %d\n"
            "#define DP_TPP %d\n#define DP_NLW %d\n#define DP_OB %d\n#define DP_PAIRPREC %d\n#define DP_CMASK %d\n",
-           c.D, c.mode, c.ntypes, c.even ? 1 : 0, c.rs, c.tpp, c.nlw, c.ob, c.pairprec, c.cmask);
+           c.D, c.mode, c.ntypes, (int)srcs.size(), c.rs, c.tpp, c.nlw, c.ob, c.pairprec, c.cmask);
   src += buf;
+  if (c.hmax > 0 && c.omax > 0) {
+    snprintf(buf, sizeof(buf), "#define DP_TS_HALO %d\n#define DP_TS_HOME %d\n", c.hmax, c.omax);
+    src += buf;
+  }
   src += "#include \"dp_device.h\"\n";
-  src += device_src;
-  src += "\n";
-  src += dp_src_pair;
+  std::string pre, pot, post;
+  split_pair_source(pre, pot, post);
+  src += pre;
+  std::string call = "#define DP_POT_CALL(pot, ...) switch (pot) {";
+  for (size_t k = 0; k < srcs.size(); ++k) {
+    snprintf(buf, sizeof(buf), "namespace dp_p%d {\n#define DP_EVEN %d\n", (int)k, evens[k] ? 1 : 0);
+    src += buf;
+    src += srcs[k];
+    src += "\n";
+    src += pot;
+    src += "#undef DP_EVEN\n}\n";
+    snprintf(buf, sizeof(buf), " case %d: { using namespace dp_p%d; __VA_ARGS__; } break;", (int)k, (int)k);
+    call += buf;
+  }
+  call += " }\n";
+  src += call;
+  src += post;
   return src;
+}
+
+std::string pair_source(const PairConfig& c, const char* device_src) {
+  return pair_source(c, std::vector<std::string>{device_src}, std::vector<int>{c.even});
 }
 
 // hiprtc compile for gfx950 (no device needed), memoised per process.
@@ -218,10 +255,10 @@ struct dp_ctx {
   std::vector<double> box, cutoff, skin, dt, T, tau;
   bool have_box = false, have_nl_params = false, have_integ = false;
   std::vector<DpRep> reps_h;
-  DpPairTab tab_h;
+  std::vector<DpPairTab> tabs_h;  // one per potential
+  std::vector<i32> rep_pot;       // potential of each replica (empty: all use potential 0)
   int ntypes = 1;
   bool have_pot = false;
-  double pot_rcut = 1.0;
   bool reps_dirty = true, tab_dirty = true;
 
   // device memory
@@ -253,8 +290,9 @@ struct dp_ctx {
   int tile_grid = 8;
   size_t lds_pair = 0, lds_emit = 0;
   // potential (compiled lazily once the plan is known)
-  std::string pot_src;
-  int pot_even = 0, pair_prec = 64;
+  std::vector<std::string> pot_srcs;  // generated device functions, one per potential
+  std::vector<int> pot_evens;
+  int pair_prec = 64;
   std::string mod_key;
   char* scratch = nullptr;
   size_t scratch_bytes = 0;
@@ -420,6 +458,7 @@ int refresh_reps(dp_ctx* ctx) {
     P.p0 = ctx->offsets[r];
     P.n = ctx->offsets[r + 1] - ctx->offsets[r];
     P.order = ctx->order;
+    P.pot = ((int)ctx->rep_pot.size() == ctx->R) ? ctx->rep_pot[r] : 0;
     const double cut = ctx->cutoff[r];
     i64 nc[DP_DMAX];
     i64 ncell = 1;
@@ -718,45 +757,47 @@ int refresh_reps(dp_ctx* ctx) {
 int refresh_tab(dp_ctx* ctx) {
   if (!ctx->tab_dirty) return DP_OK;
   if (!ctx->have_pot) return ctx->fail(DP_ERR_STATE, "no potential loaded (dp_load_potential)");
-  DpPairTab& T = ctx->tab_h;
   const int nt = ctx->ntypes;
   const double qs = std::ldexp(1.0, ctx->qbits);
   const double e32 = std::ldexp(1.0, -24);
-  T.qscale = 1.0 / qs;
-  T.qscale2 = 1.0 / (qs * qs);
-  T.qinv = qs;
-  for (int a = 0; a < nt; ++a)
-    for (int b = 0; b < nt; ++b) {
-      const int k = a * nt + b;
-      const double s = T.sigma[k], e = T.eps[k];
-      T.inv_sigma2[k] = 1.0 / (s * s);
-      T.eps_over_sigma[k] = e / s;
-      T.inv_sigma2_qf[k] = (float)(T.inv_sigma2[k] / (qs * qs));
-      T.eps_over_sigma_qf[k] = (float)(T.eps_over_sigma[k] / qs);
-      T.eps_f[k] = (float)e;
-      T.sigma_f[k] = (float)s;
-      const double t = T.cut_thresh[k];
-      if (!std::isfinite(t)) {
-        T.guard_lo_q[k] = INFINITY;
-        T.guard_hi_q[k] = INFINITY;
-        continue;
+  for (DpPairTab& T : ctx->tabs_h) {
+    T.qscale = 1.0 / qs;
+    T.qscale2 = 1.0 / (qs * qs);
+    T.qinv = qs;
+    for (int a = 0; a < nt; ++a)
+      for (int b = 0; b < nt; ++b) {
+        const int k = a * nt + b;
+        const double s = T.sigma[k], e = T.eps[k];
+        T.inv_sigma2[k] = 1.0 / (s * s);
+        T.eps_over_sigma[k] = e / s;
+        T.inv_sigma2_qf[k] = (float)(T.inv_sigma2[k] / (qs * qs));
+        T.eps_over_sigma_qf[k] = (float)(T.eps_over_sigma[k] / qs);
+        T.eps_f[k] = (float)e;
+        T.sigma_f[k] = (float)s;
+        const double t = T.cut_thresh[k];
+        if (!std::isfinite(t)) {
+          T.guard_lo_q[k] = INFINITY;
+          T.guard_hi_q[k] = INFINITY;
+          continue;
+        }
+        if (t < 0) {
+          T.guard_lo_q[k] = -1.0f;
+          T.guard_hi_q[k] = -1.0f;
+          continue;
+        }
+        const double tq = t * qs * qs;
+        double g = 3.0 * std::sqrt((double)ctx->D) * std::sqrt(tq) + 2.25 * ctx->D + 8.0 * e32 * tq;
+        g *= 4.0;  // safety factor
+        float lo = (float)(tq - g), hi = (float)(tq + g);
+        lo = std::nextafter(lo, -INFINITY);
+        hi = std::nextafter(hi, INFINITY);
+        if (lo < 0) lo = 0;
+        T.guard_lo_q[k] = lo;
+        T.guard_hi_q[k] = hi;
       }
-      if (t < 0) {
-        T.guard_lo_q[k] = -1.0f;
-        T.guard_hi_q[k] = -1.0f;
-        continue;
-      }
-      const double tq = t * qs * qs;
-      double g = 3.0 * std::sqrt((double)ctx->D) * std::sqrt(tq) + 2.25 * ctx->D + 8.0 * e32 * tq;
-      g *= 4.0;  // safety factor
-      float lo = (float)(tq - g), hi = (float)(tq + g);
-      lo = std::nextafter(lo, -INFINITY);
-      hi = std::nextafter(hi, INFINITY);
-      if (lo < 0) lo = 0;
-      T.guard_lo_q[k] = lo;
-      T.guard_hi_q[k] = hi;
-    }
-  HIPCHK(hipMemcpyAsync(ctx->tab_d, &ctx->tab_h, sizeof(DpPairTab), hipMemcpyHostToDevice, ctx->st));
+  }
+  HIPCHK(hipMemcpyAsync(ctx->tab_d, ctx->tabs_h.data(), sizeof(DpPairTab) * ctx->tabs_h.size(),
+                        hipMemcpyHostToDevice, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   ctx->tab_dirty = false;
   return DP_OK;
@@ -765,9 +806,16 @@ int refresh_tab(dp_ctx* ctx) {
 // Compile (or fetch) the pair module for the current plan.
 int ensure_module(dp_ctx* ctx) {
   if (!ctx->have_pot) return ctx->fail(DP_ERR_STATE, "no potential loaded (dp_load_potential)");
-  PairConfig c{ctx->D, ctx->mode, ctx->ntypes, ctx->pot_even, ctx->rs, ctx->tpp, ctx->nlw, ctx->ob, ctx->pair_prec,
-               ctx->cmask};
-  std::string src = pair_source(c, ctx->pot_src.c_str());
+  PairConfig c{ctx->D, ctx->mode, ctx->ntypes, 0, ctx->rs, ctx->tpp, ctx->nlw, ctx->ob, ctx->pair_prec, ctx->cmask};
+  // tile tables in LDS sized for the plan's largest tile (rounded up to 16 cells)
+  int hmax = 1, omax = 1;
+  for (const auto& P : ctx->reps_h) {
+    hmax = std::max(hmax, (int)P.nhalo);
+    omax = std::max(omax, (int)P.nhome);
+  }
+  c.hmax = std::min((hmax + 15) / 16 * 16, DP_HALO_MAX);
+  c.omax = std::min((omax + 15) / 16 * 16, DP_HOME_MAX);
+  std::string src = pair_source(c, ctx->pot_srcs, ctx->pot_evens);
   if (ctx->mod && src == ctx->mod_key) return DP_OK;
   std::vector<char> code;
   std::string log;
@@ -1249,6 +1297,30 @@ DP_API int dp_compile_check_tuned(int dim, int precision, int ntypes, int even, 
   return rc;
 }
 
+DP_API int dp_compile_check_potentials(int dim, int precision, int ntypes, int npot, const char* const* device_srcs,
+                                       const int* even, int pair_precision, char* log, int64_t log_len,
+                                       int64_t* code_size) {
+  if (dim < 1 || dim > DP_DMAX || !device_srcs || !even) return DP_ERR_ARG;
+  if (ntypes < 1 || ntypes > DP_MAXTYPES || npot < 1 || npot > DP_MAXPOT) return DP_ERR_ARG;
+  std::vector<std::string> sv;
+  std::vector<int> ev;
+  for (int p = 0; p < npot; ++p) {
+    if (!device_srcs[p]) return DP_ERR_ARG;
+    sv.emplace_back(device_srcs[p]);
+    ev.push_back(even[p] ? 1 : 0);
+  }
+  std::string src = pair_source(default_config(dim, precision, ntypes, 0, pair_precision), sv, ev);
+  std::vector<char> code;
+  std::string lg;
+  int rc = compile_cached(src, code, lg);
+  if (log && log_len > 0) {
+    std::strncpy(log, lg.c_str(), (size_t)log_len - 1);
+    log[log_len - 1] = '\0';
+  }
+  if (code_size) *code_size = (int64_t)code.size();
+  return rc;
+}
+
 DP_API int dp_compile_code(int dim, int precision, int ntypes, int even, const char* device_src, int pair_precision,
                            char* out, int64_t out_len, int64_t* code_size) {
   if (dim < 1 || dim > DP_DMAX || !device_src) return DP_ERR_ARG;
@@ -1304,7 +1376,6 @@ DP_API int dp_create(dp_ctx** out, int device, int dim, int64_t n_total, int n_r
   ctx->R = n_replicas;
   ctx->mode = precision;
   ctx->rs = rs_for_dim(dim);
-  std::memset(&ctx->tab_h, 0, sizeof(ctx->tab_h));
   auto bail = [&](int rc) {
     g_create_error = ctx->err;
     dp_destroy(ctx);
@@ -1326,7 +1397,7 @@ DP_API int dp_create(dp_ctx** out, int device, int dim, int64_t n_total, int n_r
   }
   const i64 N = n_total, D = dim;
   CCHK(dalloc(ctx, ctx->reps_d, (size_t)n_replicas));
-  CCHK(dalloc(ctx, ctx->tab_d, 1));
+  CCHK(dalloc(ctx, ctx->tab_d, DP_MAXPOT));
   CCHK(dalloc(ctx, ctx->ctl_d, 1));
   CCHK(dalloc(ctx, ctx->nlupd_d, (size_t)n_replicas));
   CCHK(dalloc(ctx, ctx->rflag, (size_t)2 * n_replicas));
@@ -1484,44 +1555,83 @@ DP_API int dp_set_neighbor_params(dp_ctx* ctx, const double* cutoff, const doubl
   return DP_OK;
 }
 
-DP_API int dp_load_potential(dp_ctx* ctx, const char* device_src, int even, double r_cut, double u_rc, int ntypes,
-                             const double* sigma, const double* eps, int pair_precision) {
-  if (!ctx || !device_src || !sigma || !eps) return DP_ERR_ARG;
+namespace {
+// Potentials k = 0..npot-1 (generated device functions srcs[k], even[k],
+// r_cut[k], u(r_cut) u_rc[k], sigma / eps tables [k][ntypes][ntypes]);
+// replica r uses potential rep_pot[r] (null: all use 0).
+int load_potentials(dp_ctx* ctx, int npot, const char* const* srcs, const int* even, const double* r_cut,
+                    const double* u_rc, int ntypes, const double* sigma, const double* eps, int pair_precision,
+                    const int32_t* rep_pot) {
   (void)hipSetDevice(ctx->dev);
+  if (npot < 1 || npot > DP_MAXPOT) return ctx->fail(DP_ERR_ARG, "number of potentials must be in 1..%d", DP_MAXPOT);
   if (ntypes < 1 || ntypes > DP_MAXTYPES) return ctx->fail(DP_ERR_ARG, "ntypes must be in 1..%d", DP_MAXTYPES);
   if (pair_precision != 0 && pair_precision != 1) return ctx->fail(DP_ERR_ARG, "pair_precision must be 0 or 1");
-  for (int k = 0; k < ntypes * ntypes; ++k)
+  const int ntt = ntypes * ntypes;
+  for (int k = 0; k < npot * ntt; ++k)
     if (!(sigma[k] > 0.0)) return ctx->fail(DP_ERR_ARG, "sigma must be positive");
-  // compile now (with the current plan) so that errors surface here
+  for (int p = 0; p < npot; ++p)
+    if (!srcs[p]) return ctx->fail(DP_ERR_ARG, "potential %d has no source", p);
+  std::vector<i32> rp((size_t)ctx->R, 0);
+  if (rep_pot)
+    for (int r = 0; r < ctx->R; ++r) {
+      if (rep_pot[r] < 0 || rep_pot[r] >= npot)
+        return ctx->fail(DP_ERR_ARG, "replica %d: potential index %d out of range", r, rep_pot[r]);
+      rp[r] = rep_pot[r];
+    }
+  std::vector<std::string> sv;
+  std::vector<int> ev;
+  for (int p = 0; p < npot; ++p) {
+    sv.emplace_back(srcs[p]);
+    ev.push_back(even[p] ? 1 : 0);
+  }
+  // compile now (with the default plan) so that errors surface here
   {
-    PairConfig c = default_config(ctx->D, ctx->mode, ntypes, even, pair_precision);
-    std::string src = pair_source(c, device_src);
+    PairConfig c = default_config(ctx->D, ctx->mode, ntypes, 0, pair_precision);
+    std::string src = pair_source(c, sv, ev);
     std::vector<char> code;
     std::string log;
     if (compile_cached(src, code, log) != DP_OK)
       return ctx->fail(DP_ERR_RTC, "hiprtc compile failed: %s", log.c_str());
   }
-  ctx->pot_src = device_src;
-  ctx->pot_even = even ? 1 : 0;
+  ctx->pot_srcs = sv;
+  ctx->pot_evens = ev;
   ctx->pair_prec = pair_precision ? 32 : 64;
   release_graph(ctx);
-  DpPairTab& T = ctx->tab_h;
-  std::memset(&T, 0, sizeof(T));
   if (ntypes != ctx->ntypes) ctx->reps_dirty = true;  // staged records carry the type with several types
+  if (rp != ctx->rep_pot) ctx->reps_dirty = true;     // DpRep::pot
+  ctx->rep_pot = rp;
   ctx->ntypes = ntypes;
-  for (int a = 0; a < ntypes; ++a)
-    for (int b = 0; b < ntypes; ++b) {
-      const int k = a * ntypes + b;
-      T.sigma[k] = sigma[k];
-      T.eps[k] = eps[k];
-      T.cut_thresh[k] = cut_threshold(sigma[k], r_cut);
+  ctx->tabs_h.assign((size_t)npot, DpPairTab{});
+  for (int p = 0; p < npot; ++p) {
+    DpPairTab& T = ctx->tabs_h[p];
+    std::memset(&T, 0, sizeof(T));
+    for (int k = 0; k < ntt; ++k) {
+      T.sigma[k] = sigma[(size_t)p * ntt + k];
+      T.eps[k] = eps[(size_t)p * ntt + k];
+      T.cut_thresh[k] = cut_threshold(T.sigma[k], r_cut[p]);
     }
-  T.r_cut = r_cut;
-  T.u_rc = u_rc;
-  ctx->pot_rcut = r_cut;
+    T.r_cut = r_cut[p];
+    T.u_rc = u_rc[p];
+  }
   ctx->have_pot = true;
   ctx->tab_dirty = true;
   return DP_OK;
+}
+}  // namespace
+
+DP_API int dp_load_potential(dp_ctx* ctx, const char* device_src, int even, double r_cut, double u_rc, int ntypes,
+                             const double* sigma, const double* eps, int pair_precision) {
+  if (!ctx || !device_src || !sigma || !eps) return DP_ERR_ARG;
+  const char* srcs[1] = {device_src};
+  const int ev[1] = {even};
+  return load_potentials(ctx, 1, srcs, ev, &r_cut, &u_rc, ntypes, sigma, eps, pair_precision, nullptr);
+}
+
+DP_API int dp_load_potentials(dp_ctx* ctx, int npot, const char* const* device_srcs, const int* even,
+                              const double* r_cut, const double* u_rc, int ntypes, const double* sigma,
+                              const double* eps, int pair_precision, const int32_t* rep_pot) {
+  if (!ctx || !device_srcs || !even || !r_cut || !u_rc || !sigma || !eps || !rep_pot) return DP_ERR_ARG;
+  return load_potentials(ctx, npot, device_srcs, even, r_cut, u_rc, ntypes, sigma, eps, pair_precision, rep_pot);
 }
 
 namespace {

@@ -5,7 +5,7 @@
 //   #define DP_D <dim>          spatial dimension (1..4)
 //   #define DP_MODE <0|1>       0 = FP64 parity mode, 1 = mixed precision
 //   #define DP_NTYPES <n>       number of particle types
-//   #define DP_EVEN <0|1>       1 if f(x)/x and u(x) are functions of x^2
+//   #define DP_NPOT <n>         number of potentials (replicas choose one each)
 //   #define DP_RS <n>           ints per mixed-precision record
 //   #define DP_TPP <n>          lanes per particle (1, 2, 4, 8, 16)
 //   #define DP_NLW <16|32>      neighbour-entry width in bits
@@ -13,7 +13,8 @@
 //   #define DP_PAIRPREC <32|64> mixed mode: precision of the in-range pair terms
 //   #define DP_CMASK <mask>     dimensions with a single cell (minimum image in
 //                               the pair loop); 0 in the usual case
-// followed by the SymPy-generated device functions
+// and, per potential k, namespace dp_p<k> holding DP_EVEN (1 if f(x)/x and
+// u(x) are functions of x^2) and the SymPy-generated device functions
 //   template<class T> T dp_pot_u(T x);   u(x)
 //   template<class T> T dp_pot_f(T x);   f(x) = -u'(x)
 //   template<class T> T dp_pot_c(T x);   u''(x)
@@ -65,44 +66,6 @@ struct DpPC {
   float ksf, kef;        // FP32 terms: 2^-2q / sigma^2, eps / sigma * 2^-q
   float epsf, sigf;
 };
-
-#ifndef DP_GS_SCALE
-#define DP_GS_SCALE 1.0  // no scaled form generated (odd potentials)
-#endif
-// Mixed mode evaluates the force term as DP_GS_SCALE * dp_pot_gsn(s): the
-// constant goes into ke / kef here.
-#if DP_EVEN
-#define DP_GSN(T, s) dp_pot_gsn<T>(s)
-#define DP_GSK DP_GS_SCALE
-#else
-#define DP_GSK 1.0
-#endif
-
-__device__ __forceinline__ DpPC dp_pc(const DpPairTab& T, int tt) {
-  DpPC c;
-  c.cut = T.cut_thresh[tt];
-  c.sig = T.sigma[tt];
-  c.eps = T.eps[tt];
-  c.eos = T.eps_over_sigma[tt];
-  c.ks = T.qscale2 * T.inv_sigma2[tt];
-  c.ke = T.eps_over_sigma[tt] * T.qscale * DP_GSK;
-  c.ghi = T.guard_hi_q[tt];
-  c.glo = T.guard_lo_q[tt];
-  c.ksf = T.inv_sigma2_qf[tt];
-  c.kef = T.eps_over_sigma_qf[tt] * (float)DP_GSK;
-  c.epsf = T.eps_f[tt];
-  c.sigf = T.sigma_f[tt];
-  return c;
-}
-
-#if DP_NTYPES == 1
-#define DP_PC_DECL const DpPC pc0 = dp_pc(*a.tab, 0);
-#define DP_PC(tt) pc0
-#else
-#define DP_PC_DECL
-#define DP_PC(tt) spc[tt]
-#endif
-
 
 template <class T>
 __device__ __forceinline__ T dp_group_sum(T v) {
@@ -337,6 +300,52 @@ __device__ __forceinline__ void dp_ld_vec(const double* p, double (&v)[DP_D]) {
 #define DP_SB (DP_D <= 3 ? 3 : 1)
 #endif
 
+// ===========================================================================
+// Potential-dependent part.  The host compiles it once per potential of the
+// context, each copy inside its own namespace dp_p<k> together with that
+// potential's generated device functions and DP_EVEN; the tile kernels below
+// call the copy of the tile's replica's potential (DP_POT_CALL).
+// ===========================================================================
+//@@POTENTIAL_BEGIN
+#ifndef DP_GS_SCALE
+#define DP_GS_SCALE 1.0  // no scaled form generated (odd potentials)
+#endif
+// Mixed mode evaluates the force term as DP_GS_SCALE * dp_pot_gsn(s): the
+// constant goes into ke / kef here.
+#if DP_EVEN
+#define DP_GSN(T, s) dp_pot_gsn<T>(s)
+#define DP_GSK DP_GS_SCALE
+#else
+#define DP_GSK 1.0
+#endif
+
+__device__ __forceinline__ DpPC dp_pc(const DpPairTab& T, int tt) {
+  DpPC c;
+  c.cut = T.cut_thresh[tt];
+  c.sig = T.sigma[tt];
+  c.eps = T.eps[tt];
+  c.eos = T.eps_over_sigma[tt];
+  c.ks = T.qscale2 * T.inv_sigma2[tt];
+  c.ke = T.eps_over_sigma[tt] * T.qscale * DP_GSK;
+  c.ghi = T.guard_hi_q[tt];
+  c.glo = T.guard_lo_q[tt];
+  c.ksf = T.inv_sigma2_qf[tt];
+  c.kef = T.eps_over_sigma_qf[tt] * (float)DP_GSK;
+  c.epsf = T.eps_f[tt];
+  c.sigf = T.sigma_f[tt];
+  return c;
+}
+
+#if DP_NTYPES == 1
+#define DP_PC_DECL const DpPC pc0 = dp_pc(T, 0);
+#define DP_PC(tt) pc0
+#else
+#define DP_PC_DECL
+#define DP_PC(tt) spc[tt]
+#endif
+
+
+
 // Per-particle pass over the home particles of the staged tile.  LDS selects
 // where neighbour data come from: the staged halo (LDS = true) or global
 // memory through the tile tables (tiles whose halo exceeds the LDS capacity).
@@ -345,7 +354,7 @@ __device__ __forceinline__ void dp_ld_vec(const double* p, double (&v)[DP_D]) {
 template <int KIND, bool LDS>
 __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpTileSmem<DP_D>& S, const DpRep& P,
                                                   const void* stage, const DpPC* spc, bool& trip) {
-  const DpPairTab& T = *a.tab;
+  const DpPairTab& T = a.tab[P.pot];  // this replica's potential
   (void)spc;
   DP_PC_DECL
   const int tid = threadIdx.x;
@@ -652,6 +661,18 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
   }
 }
 
+// This potential's type-pair constants into the workgroup's LDS table.
+__device__ __forceinline__ void dp_fill_spc(DpPC* spc, const DpPairTab& T) {
+  if ((int)threadIdx.x < DP_NTYPES * DP_NTYPES) spc[threadIdx.x] = dp_pc(T, (int)threadIdx.x);
+}
+#undef DP_GSN
+#undef DP_GSK
+#undef DP_GS_SCALE
+#undef DP_PC_DECL
+#undef DP_PC
+//@@POTENTIAL_END
+// ===========================================================================
+
 template <int KIND>
 __device__ void dp_tile_run(const DpPairArgs& a) {
   if (!dp_blk_go(a.ctl, a.blk_pos)) return;  // a skipped step of a step block (uniform)
@@ -659,11 +680,9 @@ __device__ void dp_tile_run(const DpPairArgs& a) {
   extern __shared__ __align__(16) i32 stage_i[];  // mixed: cap * DP_LS ints; FP64: cap * DP_D doubles + cap types
   const int tid = threadIdx.x;
 #if DP_NTYPES > 1
-  __shared__ DpPC spc[DP_NTYPES * DP_NTYPES];
-  if (tid < DP_NTYPES * DP_NTYPES) spc[tid] = dp_pc(*a.tab, tid);
-  // visible after the first tile's table-load barrier
+  __shared__ DpPC spc[DP_NTYPES * DP_NTYPES];  // filled per tile (the tile's potential)
 #else
-  const DpPC* spc = nullptr;
+  DpPC* spc = nullptr;
 #endif
   bool trip = false;  // some particle of this lane moved beyond half the skin
   const i64 chunk = (a.ntiles + 7) / 8;
@@ -716,12 +735,16 @@ __device__ void dp_tile_run(const DpPairArgs& a) {
       }
 #endif
     }
+#if DP_NTYPES > 1
+    DP_POT_CALL(P.pot, dp_fill_spc(spc, a.tab[P.pot]));
+#endif
     __syncthreads();
     bool trip_t = false;  // a particle of this tile (replica S.rep) moved beyond half the skin
-    if (fits)
-      dp_tile_particles<KIND, true>(a, S, P, stage_i, spc, trip_t);
-    else
-      dp_tile_particles<KIND, false>(a, S, P, stage_i, spc, trip_t);
+    if (fits) {
+      DP_POT_CALL(P.pot, dp_tile_particles<KIND, true>(a, S, P, stage_i, spc, trip_t));
+    } else {
+      DP_POT_CALL(P.pot, dp_tile_particles<KIND, false>(a, S, P, stage_i, spc, trip_t));
+    }
     if (KIND == 2 || KIND == 3) {
       dp_raise_replica(trip_t, a.rflag, S.rep);
       trip |= trip_t;

@@ -121,6 +121,24 @@ class Engine:
                                         float(u_rc), nt, _ptr(sig), _ptr(eps), acc)
         self._chk(rc)
 
+    def load_potentials(self, codes, r_cuts, u_rcs, sigmas, epsilons, rep_pot, pair_precision: str = "fp64") -> None:
+        """Several potentials (PotentialCode, r_cut, u(r_cut), sigma and
+        epsilon tables of equal size), replica r using potential rep_pot[r]."""
+        npot = len(codes)
+        sig = np.ascontiguousarray(np.stack([np.atleast_2d(np.asarray(t, dtype=np.float64)) for t in sigmas]))
+        eps = np.ascontiguousarray(np.stack([np.atleast_2d(np.asarray(t, dtype=np.float64)) for t in epsilons]))
+        nt = sig.shape[1]
+        if sig.shape != (npot, nt, nt) or eps.shape != (npot, nt, nt):
+            raise ValueError("one square sigma / epsilon table of equal size per potential")
+        srcs = (ctypes.c_char_p * npot)(*[c.source.encode() for c in codes])
+        even = np.ascontiguousarray(np.array([int(c.even) for c in codes], dtype=np.int32))
+        rc = np.ascontiguousarray(np.asarray(r_cuts, dtype=np.float64).reshape(npot))
+        urc = np.ascontiguousarray(np.asarray(u_rcs, dtype=np.float64).reshape(npot))
+        rp = np.ascontiguousarray(np.asarray(rep_pot, dtype=np.int32).reshape(self.n_replicas))
+        acc = {"fp64": 0, "fp32": 1}[pair_precision]
+        self._chk(self.lib.dp_load_potentials(self.ctx, npot, srcs, _ptr(even), _ptr(rc), _ptr(urc), nt, _ptr(sig),
+                                              _ptr(eps), acc, _ptr(rp)))
+
     def set_seed(self, seed: int, step_counter: int = 0) -> None:
         self._chk(self.lib.dp_set_seed(self.ctx, int(seed) & (2**64 - 1), int(step_counter)))
 

@@ -96,6 +96,17 @@ int dp_set_neighbor_params(dp_ctx* ctx, const double* cutoff, const double* skin
 int dp_load_potential(dp_ctx* ctx, const char* device_src, int even, double r_cut, double u_rc, int ntypes,
                       const double* sigma, const double* eps, int pair_precision);
 
+/* Several potentials in one context (batched replicas of different
+ * potentials, stepped in the same launches): potential k has generated
+ * device functions device_srcs[k], even[k], r_cut[k], u(r_cut) u_rc[k] and
+ * sigma / eps tables [k][ntypes][ntypes]; replica r uses potential
+ * rep_pot[r].  At most 8 potentials.  No counterpart in the reference (one
+ * Simulation, one potential: potential.py:43-66, Simulation.py:403-525 per
+ * replica). */
+int dp_load_potentials(dp_ctx* ctx, int npot, const char* const* device_srcs, const int* even, const double* r_cut,
+                       const double* u_rc, int ntypes, const double* sigma, const double* eps, int pair_precision,
+                       const int32_t* rep_pot);
+
 /* Upload / download one per-particle field in the reference layout.
  * Simulation attribute assignment / access (Simulation.py:101-147). */
 int dp_set_field(dp_ctx* ctx, int field, const void* src);
@@ -203,6 +214,10 @@ int dp_compile_check(int dim, int precision, int ntypes, int even, const char* d
 int dp_compile_check_tuned(int dim, int precision, int ntypes, int even, const char* device_src,
                            int pair_precision, int threads_per_particle, int entry_bits, int collapsed_mask,
                            char* log, int64_t log_len, int64_t* code_size);
+/* Same check for the kernels of several potentials in one context (as
+ * dp_load_potentials compiles them). */
+int dp_compile_check_potentials(int dim, int precision, int ntypes, int npot, const char* const* device_srcs,
+                                const int* even, int pair_precision, char* log, int64_t log_len, int64_t* code_size);
 /* Same compile; copies the gfx950 code object into out when out_len is large
  * enough (code_size receives the size). */
 int dp_compile_code(int dim, int precision, int ntypes, int even, const char* device_src, int pair_precision,

@@ -142,3 +142,41 @@ def test_block_rebuild_modes_give_identical_trajectories(precision):
                      s.number_of_neighbor_list_updates))
     for x, v, u in runs[1:]:
         assert np.array_equal(x, runs[0][0]) and np.array_equal(v, runs[0][1]) and u == runs[0][2]
+
+
+POTS = {"lj": ("4*(r**-12-r**-6)", 2.5), ("ipl", 12): ("r**-12", 1.5), "harm2": ("(1-r)**2", 1.0)}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_of_different_potentials_against_c_oracle(precision):
+    """Replicas of three potentials (LJ, r^-12, (1-r)^2) stepped in the same
+    launches, each computing with its own potential: forces / energy / virial
+    of every replica against the C oracle with that potential."""
+    from dompap_amd import ReplicaBatch
+    from oracle import c_oracle as C
+    plan = [("lj", 5, 0.85, 1.0), ("harm2", 6, 1.2, 0.01), (("ipl", 12), 5, 1.0, 1.0), ("lj", 4, 0.95, 0.7),
+            (("ipl", 12), 4, 0.9, 1.5), ("harm2", 5, 1.1, 0.02), ("lj", 6, 0.8, 1.2)]
+    sims = []
+    for i, (pot, cells, rho, T) in enumerate(plan):
+        s = _replica(precision, cells, rho, T, 500 + i)
+        expr, rc = POTS[pot]
+        s.set_pair_potential(expr, r_cut=rc)
+        s.set_neighbor_list(skin=0.4, max_number_of_neighbors=160)
+        sims.append(s)
+    b = ReplicaBatch(sims)
+    assert sorted(set(b.rep_pot.tolist())) == [0, 1, 2]
+    b.run(30)
+    xs = b.get_field("positions")
+    Fb, Eb, Wb = b.get_forces(), b.get_potential_energy(), b.get_virial()
+    tol_f, tol_e = (1e-10, 1e-10) if precision == "fp64" else (1e-5, 1e-6)
+    for r, (pot, cells, rho, T) in enumerate(plan):
+        x = xs[r]
+        box = np.asarray(sims[r].box_vectors, dtype=np.float64)
+        rc = POTS[pot][1]
+        nl, _ = C.neighbor_list(x, box, rc + 0.4, 160, "cells")
+        o = C.pair(x, box, nl, pot, rc, energy=True, virial=True)
+        n = x.shape[0]
+        assert np.abs(Fb[r] - o["forces"]).max() / _frms(o["forces"]) < tol_f, r
+        assert abs(Eb[r] - o["energy"]) < tol_e * max(abs(o["energy"]), n), r
+        assert abs(Wb[r] - o["virial"]) < 10 * tol_e * max(abs(o["virial"]), n), r
+    b.close()
