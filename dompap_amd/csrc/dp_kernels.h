@@ -1147,7 +1147,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* 
                                                                const DpRep* __restrict__ reps, int R,
                                                                i64 ntiles, int K, int cap, E* nl, i32* nlcnt,
                                                                i32* tiletab, int tstride, DpCtl* ctl, int force,
-                                                               int lanes, const i32* __restrict__ rb) {
+                                                               int lanes, const i32* __restrict__ todo) {
   if (!force && !ctl->rebuild) return;
   // non-finite positions were binned into arbitrary cells: skip the (then
   // unbounded) emit; the error is reported after the step block
@@ -1160,9 +1160,11 @@ __global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* 
   const int wave = tid >> 6;
   const int nwaves = blockDim.x >> 6;
   int st_count = 0, st_cell = 0, st_halo = 0;  // running maxima (per wavefront / thread 0)
-  for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    // tiles of replicas that keep their list (uniform over the workgroup)
-    if (!force && !rb[dp_tile_replica(reps, R, t)]) continue;
+  // the tiles to emit: all of them, or (batches) the list of the rebuilding
+  // replicas' tiles built by k_emit_list
+  const i64 ntodo = todo ? (i64)ctl->emit_ntodo : ntiles;
+  for (i64 i = blockIdx.x; i < ntodo; i += gridDim.x) {
+    const i64 t = todo ? (i64)todo[i] : i;
     dp_tile_setup<D>(S, reps, R, cstart, t, cap);
     const DpRep& P = reps[__builtin_amdgcn_readfirstlane(S.rep)];  // uniform: scalar loads
     const bool fits = S.fits != 0;
@@ -1202,6 +1204,44 @@ __global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* 
   }
 }
 
+// The emit's tile list of a batch: the tiles of the replicas that rebuild
+// this step, replica by replica (one workgroup; a block scan of the
+// rebuilding replicas' tile counts, then each thread writes its replica's
+// tiles).  Outside rebuilds the list is empty.
+__global__ void __launch_bounds__(256) k_emit_list(const DpRep* __restrict__ reps, int R, const i32* __restrict__ rb,
+                                                   i32* todo, DpCtl* ctl) {
+  __shared__ int sh[256];
+  __shared__ int carry;
+  const int tid = threadIdx.x;
+  if (!ctl->rebuild) {
+    if (tid == 0) ctl->emit_ntodo = 0;
+    return;
+  }
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < R; base += 256) {
+    const int r = base + tid;
+    const int cnt = (r < R && rb[r]) ? (int)reps[r].ntiles : 0;
+    sh[tid] = cnt;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      const int v = tid >= off ? sh[tid - off] : 0;
+      __syncthreads();
+      sh[tid] += v;
+      __syncthreads();
+    }
+    const int first = carry + sh[tid] - cnt;
+    if (cnt > 0) {
+      const int t0 = (int)reps[r].t0;
+      for (int j = 0; j < cnt; ++j) todo[first + j] = t0 + j;
+    }
+    __syncthreads();
+    if (tid == 255) carry += sh[255];
+    __syncthreads();
+  }
+  if (tid == 0) ctl->emit_ntodo = carry;
+}
+
 // Emit for the tiles marked by k_emit_tile (halo larger than the LDS record
 // area): the same emit reading records from global memory.
 template <int D, typename E>
@@ -1209,16 +1249,17 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile_far(const double*
                                                                    const i32* __restrict__ cellpack, double qinv,
                                                                    const DpRep* __restrict__ reps, i64 ntiles, int K,
                                                                    E* nl, i32* nlcnt, const i32* tiletab, int tstride,
-                                                                   DpCtl* ctl, int force, int R,
-                                                                   const i32* __restrict__ rb) {
+                                                                   DpCtl* ctl, int force,
+                                                                   const i32* __restrict__ todo) {
   if (!force && !ctl->rebuild) return;
   if (ctl->nonfinite) return;
   __shared__ DpTileSmem<D> S;
   int st_count = 0;
-  for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  const i64 ntodo = todo ? (i64)ctl->emit_ntodo : ntiles;
+  for (i64 i = blockIdx.x; i < ntodo; i += gridDim.x) {
+    const i64 t = todo ? (i64)todo[i] : i;
     const i32* tt = tiletab + t * (i64)tstride;
     if (tt[6] == 0) continue;  // uniform over the workgroup
-    if (!force && !rb[dp_tile_replica(reps, R, t)]) continue;
     dp_tile_load<D>(S, tt, 0);
     const DpRep& P = reps[__builtin_amdgcn_readfirstlane(S.rep)];  // uniform: scalar loads
     // st_off is not part of the stored tables

@@ -307,6 +307,8 @@ struct dp_ctx {
   // state flags
   bool nl_built = false;
   i32* tiletab = nullptr;  // per-tile tables written at every build
+  i32* todo = nullptr;     // [ntiles] the emit's tile list (batches)
+  i64 todo_n = 0;
   int tstride = 0;
   size_t tiletab_bytes = 0;
   bool nl_valid = true;                 // false when the list buffer was reallocated
@@ -715,6 +717,11 @@ int refresh_reps(dp_ctx* ctx) {
       release_graph(ctx);
     }
   }
+  if (t0 != ctx->todo_n || !ctx->todo) {
+    CHK(dalloc(ctx, ctx->todo, (size_t)std::max<i64>(t0, 1)));
+    ctx->todo_n = t0;
+    release_graph(ctx);  // captured emits carry the list's address
+  }
   // rows padded to a multiple of 8 entries (the pair kernels read 8 bytes at a time)
   const size_t nl_entries = (size_t)ctx->N * (size_t)((ctx->K + 7) & ~7);
   if (nl_entries >= ((size_t)1 << 32))  // the emit indexes the list with 32-bit entry offsets
@@ -901,12 +908,22 @@ int launch_flat(dp_ctx* ctx, hipFunction_t f, DpPairArgs& a) {
 template <int D, typename E>
 int launch_emit_te(dp_ctx* ctx, int force) {
   const double qinv = std::ldexp(1.0, ctx->qbits);
-  k_emit_tile<D, E><<<ctx->tile_grid, DP_TILE_THREADS, ctx->lds_emit, ctx->st>>>(
+  // batches: only the rebuilding replicas' tiles, listed on the device, a
+  // bounded grid looping over the list (a grid of one workgroup per tile
+  // would spend most of a partial rebuild starting and retiring idle ones)
+  const bool listed = !force && ctx->R > 1;
+  int grid = ctx->tile_grid;
+  if (listed) {
+    k_emit_list<<<1, 256, 0, ctx->st>>>(ctx->reps_d, ctx->R, ctx->rflag + ctx->R, ctx->todo, ctx->ctl_d);
+    grid = std::min(grid, 1024);
+  }
+  const i32* todo = listed ? ctx->todo : nullptr;
+  k_emit_tile<D, E><<<grid, DP_TILE_THREADS, ctx->lds_emit, ctx->st>>>(
       ctx->xref, ctx->cellpack, qinv, ctx->cstart, ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, ctx->cap_emit,
-      (E*)ctx->nl, ctx->nlcnt, ctx->tiletab, ctx->tstride, ctx->ctl_d, force, ctx->emit_lanes, ctx->rflag + ctx->R);
-  k_emit_tile_far<D, E><<<ctx->tile_grid, DP_TILE_THREADS, 0, ctx->st>>>(
+      (E*)ctx->nl, ctx->nlcnt, ctx->tiletab, ctx->tstride, ctx->ctl_d, force, ctx->emit_lanes, todo);
+  k_emit_tile_far<D, E><<<grid, DP_TILE_THREADS, 0, ctx->st>>>(
       ctx->xref, ctx->cellpack, qinv, ctx->reps_d, ctx->ntiles, ctx->K, (E*)ctx->nl, ctx->nlcnt, ctx->tiletab,
-      ctx->tstride, ctx->ctl_d, force, ctx->R, ctx->rflag + ctx->R);
+      ctx->tstride, ctx->ctl_d, force, todo);
   HIPCHK(hipGetLastError());
   return DP_OK;
 }
@@ -1488,7 +1505,7 @@ DP_API int dp_destroy(dp_ctx* ctx) {
   dfree(ctx->W); dfree(ctx->LAP); dfree(ctx->xi); dfree(ctx->img); dfree(ctx->type); dfree(ctx->orig);
   dfree(ctx->rep); dfree(ctx->cellpack); dfree(ctx->cell_of); dfree(ctx->rank); dfree(ctx->tmp);
   dfree(ctx->perm); dfree(ctx->nlcnt); dfree(ctx->ccount); dfree(ctx->cstart);
-  dfree(ctx->nl); dfree(ctx->tiletab); dfree(ctx->scratch); dfree(ctx->stage);
+  dfree(ctx->nl); dfree(ctx->tiletab); dfree(ctx->todo); dfree(ctx->scratch); dfree(ctx->stage);
   if (ctx->ctl_h) (void)hipHostFree(ctx->ctl_h);
   if (ctx->st) (void)hipStreamDestroy(ctx->st);
   delete ctx;

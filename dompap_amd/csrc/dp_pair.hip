@@ -351,7 +351,7 @@ __device__ __forceinline__ DpPC dp_pc(const DpPairTab& T, int tt) {
 // memory through the tile tables (tiles whose halo exceeds the LDS capacity).
 // KIND: 0 forces, 1 observables (forces + per-particle energy / virial /
 // Laplacian), 2 fused force + G-JF, 3 fused force + leap-frog.
-template <int KIND, bool LDS>
+template <int KIND, bool LDS, int CM>
 __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpTileSmem<DP_D>& S, const DpRep& P,
                                                   const void* stage, const DpPC* spc, bool& trip) {
   const DpPairTab& T = a.tab[P.pot];  // this replica's potential
@@ -415,7 +415,7 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
 #pragma unroll
         for (int d = 0; d < DP_D; ++d) {
           i32 dd = (i32)((u32)ri.v[d] - (u32)rj[c].v[d]);
-          if (DP_CMASK & (1 << d)) {
+          if (CM & (1 << d)) {
             // single cell along d (offsets from the box centre): minimum image
             // (a no-op for other replicas, whose displacements are < L/2)
             const i32 h = P.Lq[d] >> 1;
@@ -740,10 +740,18 @@ __device__ void dp_tile_run(const DpPairArgs& a) {
 #endif
     __syncthreads();
     bool trip_t = false;  // a particle of this tile (replica S.rep) moved beyond half the skin
-    if (fits) {
-      DP_POT_CALL(P.pot, dp_tile_particles<KIND, true>(a, S, P, stage_i, spc, trip_t));
+    // replicas with a single cell along some dimension take the copy with the
+    // minimum image in the pair loop; the others (usually all) skip it
+    if (DP_CMASK != 0 && P.collapsed_mask != 0) {
+      if (fits) {
+        DP_POT_CALL(P.pot, dp_tile_particles<KIND, true, DP_CMASK>(a, S, P, stage_i, spc, trip_t));
+      } else {
+        DP_POT_CALL(P.pot, dp_tile_particles<KIND, false, DP_CMASK>(a, S, P, stage_i, spc, trip_t));
+      }
+    } else if (fits) {
+      DP_POT_CALL(P.pot, dp_tile_particles<KIND, true, 0>(a, S, P, stage_i, spc, trip_t));
     } else {
-      DP_POT_CALL(P.pot, dp_tile_particles<KIND, false>(a, S, P, stage_i, spc, trip_t));
+      DP_POT_CALL(P.pot, dp_tile_particles<KIND, false, 0>(a, S, P, stage_i, spc, trip_t));
     }
     if (KIND == 2 || KIND == 3) {
       dp_raise_replica(trip_t, a.rflag, S.rep);
