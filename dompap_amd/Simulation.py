@@ -328,6 +328,7 @@ class Simulation:
             self.__dict__["_integ_dirty"] = False
         if self.__dict__.get("_tuning_dirty", True):
             eng.set_tuning(*self.__dict__.get("_tuning", (0, 0, -1)))
+            eng.set_cell_scale(self.__dict__.get("_cell_scale", 1.0))
             self.__dict__["_tuning_dirty"] = False
         bad = self.__dict__.setdefault("_bad_fields", {})
         for f in ("positions", "velocities", "betas", "image_positions", "masses"):
@@ -669,13 +670,16 @@ class Simulation:
         if temperature_damping_time is not None:
             self.temperature_damping_time = np.float64(temperature_damping_time)
 
-    def set_kernel_tuning(self, threads_per_particle: int = 0, tile_particles: int = 0, fused: int = -1) -> None:
+    def set_kernel_tuning(self, threads_per_particle: int = 0, tile_particles: int = 0, fused: int = -1,
+                          cell_scale: float = 1.0) -> None:
         """GPU kernel shape (no counterpart in the reference; set by
         dompap_amd.tools.autotune): lanes per particle (0: planner), home
         particles per tile (0: planner), fused force + integrate kernel (1),
-        split (0) or planner's choice (-1).  Results do not depend on it
-        beyond summation order."""
+        split (0) or planner's choice (-1), cells at least cell_scale x the
+        list cutoff wide.  Results do not depend on it beyond summation
+        order."""
         self.__dict__["_tuning"] = (int(threads_per_particle), int(tile_particles), int(fused))
+        self.__dict__["_cell_scale"] = float(cell_scale)
         self.__dict__["_tuning_dirty"] = True
 
     def get_kernel_plan(self) -> dict:

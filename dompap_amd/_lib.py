@@ -67,6 +67,7 @@ SIGNATURES = {
     "dp_set_seed": (_c_int, [_vp, _c_u64, _c_u64]),
     "dp_set_replica_seeds": (_c_int, [_vp, _vp]),
     "dp_set_tuning": (_c_int, [_vp, _c_int, _c_int, _c_int]),
+    "dp_set_cell_scale": (_c_int, [_vp, _c_dbl]),
     "dp_get_plan": (_c_int, [_vp, _vp, _c_int]),
     "dp_get_neighbor_list": (_c_int, [_vp, _vp]),
     "dp_get_neighbor_counts": (_c_int, [_vp, _vp]),

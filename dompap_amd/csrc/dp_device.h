@@ -162,6 +162,7 @@ struct DpPairArgs {
   i32 wrap;               // 1: wrap positions into the box after the update
   i32 blk_pos;            // position of this step in a step block, -1 outside blocks
   i32* rflag;             // [2R]: need[r] (raised by replica r's skin trigger), rebuild[r]
+  i32 unit_mass;          // 1: every mass is 1.0 (the integrators skip reading mass)
 };
 
 #ifndef DP_NO_DEVICE_HELPERS

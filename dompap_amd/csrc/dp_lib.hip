@@ -281,9 +281,11 @@ struct dp_ctx {
   int nlw = 16, ob = 11, tpp = 8, qbits = 24, cap = 1024, cap_emit = 1024;
   int emit_lanes = 0;  // 1: the emit takes one lane per home particle (cells of a few particles)
   int nlw_min = 16;    // 32 after a tile's halo outgrew 16-bit entries (kept for later builds)
+  bool unit_mass = true;  // every mass is 1.0 (as at creation)
   bool last_cell_overflow = false;  // the last check_ctl failed on a halo beyond the entry width
   // tuning (0 = automatic)
   int tune_tpp = 0, tune_tile = 0, tune_fused = -1;
+  double cell_scale = 1.0;  // cells at least cell_scale x cutoff wide (tuning; >= 1)
   int fused = 0;  // 1: fused force + integrate kernel; 0: force kernel, then integrate kernel
   int cmask = 0;  // OR of the replicas' collapsed-dimension masks
   i64 ntiles = 0;
@@ -466,7 +468,7 @@ int refresh_reps(dp_ctx* ctx) {
     i64 ncell = 1;
     for (int d = 0; d < D; ++d) {
       const double L = ctx->box[(size_t)r * D + d];
-      double q = (cut > 0.0 && std::isfinite(cut)) ? std::floor(L / cut) : 1.0;
+      double q = (cut > 0.0 && std::isfinite(cut)) ? std::floor(L / (cut * ctx->cell_scale)) : 1.0;
       if (!(q >= 3.0)) q = 1.0;
       if (q > (double)maxc) q = (double)maxc;
       nc[d] = (i64)q;
@@ -884,6 +886,7 @@ DpPairArgs pair_args(dp_ctx* ctx, int cur) {
   a.blk_pos = -1;
   a.wrap = 1;
   a.rflag = ctx->rflag;
+  a.unit_mass = ctx->unit_mass ? 1 : 0;
   return a;
 }
 
@@ -1689,6 +1692,17 @@ DP_API int dp_set_field(dp_ctx* ctx, int field, const void* src) {
     for (i64 i = 0; i < ctx->N; ++i)
       if (t[i] < 0 || t[i] >= ctx->ntypes) return ctx->fail(DP_ERR_ARG, "particle type %d out of range", t[i]);
   }
+  if (field == DP_FIELD_MASSES) {
+    // unit masses let the integrators skip the mass array (a kernel argument:
+    // captured step blocks are re-captured when it changes)
+    std::vector<double> m((size_t)ctx->N);
+    HIPCHK(hipMemcpyAsync(m.data(), ctx->stage, (size_t)ctx->N * 8, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    bool unit = true;
+    for (double v : m) unit = unit && (v == 1.0);
+    if (unit != ctx->unit_mass) release_graph(ctx);
+    ctx->unit_mass = unit;
+  }
   if (field == DP_FIELD_POSITIONS || field == DP_FIELD_XREF) ctx->pos_dirty = true;
   if (field == DP_FIELD_POSITIONS || field == DP_FIELD_TYPES) CHK(launch_records(ctx));
   HIPCHK(hipStreamSynchronize(ctx->st));
@@ -1894,6 +1908,18 @@ DP_API int dp_set_tuning(dp_ctx* ctx, int threads_per_particle, int tile_particl
   ctx->tune_fused = fused < 0 ? -1 : (fused ? 1 : 0);
   ctx->reps_dirty = true;
   release_graph(ctx);
+  return DP_OK;
+}
+
+DP_API int dp_set_cell_scale(dp_ctx* ctx, double scale) {
+  if (!ctx) return DP_ERR_ARG;
+  if (!(scale >= 1.0) || !(scale <= 4.0)) return ctx->fail(DP_ERR_ARG, "cell scale must be in [1, 4]");
+  if (scale != ctx->cell_scale) {
+    ctx->cell_scale = scale;
+    ctx->reps_dirty = true;
+    ctx->nl_built = false;  // new cells: the list is rebuilt at the next use
+    release_graph(ctx);
+  }
   return DP_OK;
 }
 

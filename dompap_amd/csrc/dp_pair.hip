@@ -587,7 +587,7 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
     {
 #pragma clang fp contract(off)
       const i64 ii = slot_i;
-      const double m = a.mass[ii];
+      const double m = a.unit_mass ? 1.0 : a.mass[ii];
       const double dt = P.dt;
       double z_own[DP_JMAX];
       if (KIND == 2) {
@@ -644,8 +644,9 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
       }
 #if DP_MODE == 1
       if (sub == 0) {
+        // slots after the coordinates: the type (0 with one type), padding
 #pragma unroll
-        for (int q = DP_D; q < DP_RS; ++q) a.xq_out[ii * DP_RS + q] = a.xq[ii * DP_RS + q];
+        for (int q = DP_D; q < DP_RS; ++q) a.xq_out[ii * DP_RS + q] = (DP_NTYPES > 1) ? a.xq[ii * DP_RS + q] : 0;
       }
 #endif
       // squared displacement summed in dimension order from the owners
@@ -811,14 +812,14 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
   for (int d = 0; d < DP_D; ++d) b0[d] = xr0[d] = 0.0;
   if (INTEG == 1) dp_ld_vec(a.beta + ii * DP_D, b0);
   if (a.wrap) dp_ld_vec(a.xref + ii * DP_D, xr0);
-  const double m = a.mass[ii];
+  const double m = a.unit_mass ? 1.0 : a.mass[ii];
   const u32 pid = (INTEG == 1) ? (u32)a.orig[ii] : 0u;
 #if DP_MODE == 1
   const u32 pack = a.wrap ? (u32)a.cellpack[ii] : 0u;
-  DpRec rec;  // the record; its slots after the coordinates (type) carry over
+  DpRec rec;  // the record; its slots after the coordinates (type; 0 with one type) carry over
 #pragma unroll
   for (int q = 0; q < DP_RS; ++q) rec.v[q] = 0;
-  if (a.wrap) rec = dp_ld_rec(a.xq + ii * DP_RS);
+  if (a.wrap && DP_NTYPES > 1) rec = dp_ld_rec(a.xq + ii * DP_RS);
 #endif
   const int r = (a.R == 1) ? 0 : a.rep[ii];
   const DpRep& P = a.reps[r];

@@ -152,6 +152,10 @@ class Engine:
         (-1 = planner's choice)."""
         self._chk(self.lib.dp_set_tuning(self.ctx, int(threads_per_particle), int(tile_particles), int(fused)))
 
+    def set_cell_scale(self, scale: float) -> None:
+        """Cells at least ``scale`` x cutoff wide (1 <= scale <= 4)."""
+        self._chk(self.lib.dp_set_cell_scale(self.ctx, float(scale)))
+
     PLAN_KEYS = ("threads_per_particle", "entry_bits", "offset_bits", "fixed_point_bits", "lds_records",
                  "lds_records_emit", "tiles", "tile_grid", "fused", "home_cells_per_tile", "halo_cells_per_tile",
                  "cells", "steps_per_block", "graph_events")

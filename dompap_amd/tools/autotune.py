@@ -1,79 +1,110 @@
 """GPU autotuner with the interface of dompap/tools/autotune.py:4-178.
 
-The reference times its neighbour-list methods, scans the skin upward from
-``smallest_skin`` in ``step_skin`` increments until a run of ``steps`` steps
-gets slower, and times its force methods.  Here every neighbour-list and
-force method string runs the same GPU kernels, so the method choice keeps
-the simulation's strings; the skin scan is the reference's; and the kernel
-shape (lanes per particle, fused or split step) is timed at the chosen skin
-and stored in the simulation (``Simulation.set_kernel_tuning``).
+What the reference tunes, and what it tunes here:
+
+* neighbour-list and force method strings: every method string runs the
+  same GPU kernels, so the simulation keeps its strings;
+* the skin (autotune.py:62-90): the same search -- the skin grows from
+  ``smallest_skin`` by ``step_skin`` as long as a timed run of ``steps``
+  steps gets faster, and stops at the first slower run or at half the
+  shortest box side; the fastest skin wins;
+* no counterpart in the reference, for the GPU: the kernel shape (lanes per
+  particle, fused or split step kernels), the cell width (cells of 1.0 or
+  more times the list cutoff: larger cells make larger tiles) and the row
+  capacity (``max_number_of_neighbors``: lowered, never raised, to 1.5 times
+  the longest row seen while tuning, in multiples of 8).  They are timed on
+  the system itself, so each system shape gets its own choice, and stored in
+  the simulation (``Simulation.set_kernel_tuning``, ``set_neighbor_list``).
 """
 from __future__ import annotations
 
 from time import perf_counter
 
+CELL_SCALES = (1.0, 1.15, 1.3)
 
-def _time_run(sim, steps):
-    tic = perf_counter()
+
+def _seconds(sim, steps):
+    start = perf_counter()
     sim.run(steps)  # returns after the device finished the steps
-    return perf_counter() - tic
+    return perf_counter() - start
+
+
+def _longest_row(sim):
+    _, _, longest = sim._device().stats()
+    return int(longest)
 
 
 def autotune(sim, steps=100, test_double_loop=True, smallest_skin=0.1, step_skin=0.1,
-             verbose=False, plot=False):
-    """Tune skin and kernel shape of ``sim`` (dompap/tools/autotune.py:4-6);
-    returns ``sim``."""
-    sim_copy = sim.copy()
-    sim_copy.step()  # initialise (compile, first list)
+             verbose=False, plot=False, tune_capacity=True):
+    """Tune skin, kernel shape, cell width and row capacity of ``sim``
+    (dompap/tools/autotune.py:4-6); returns ``sim``."""
+    probe = sim.copy()
+    probe.step()  # compiles the kernels and builds the first list
 
-    # skin scan (autotune.py:62-90): upward until a run gets slower
-    largest_allowed_skin = min(sim.box_vectors) / 2
-    skins, times, updates = [], [], []
-    skin = smallest_skin
-    old_time = float("inf")
-    while skin < largest_allowed_skin:
-        n0 = sim_copy.number_of_neighbor_list_updates
-        sim_copy.set_neighbor_list(skin=skin)
-        sim_copy.step()  # the new list and plan, outside the timed run
-        t = _time_run(sim_copy, steps)
-        skins.append(skin)
-        times.append(t)
-        updates.append(sim_copy.number_of_neighbor_list_updates - n0)
-        skin += step_skin
-        if t > old_time:
+    # -- skin: grow while the timed run gets faster ------------------------
+    limit = min(sim.box_vectors) / 2
+    tried = []  # (skin, seconds, rebuilds)
+    candidate = smallest_skin
+    previous = float("inf")
+    longest = 0
+    while candidate < limit:
+        before = probe.number_of_neighbor_list_updates
+        probe.set_neighbor_list(skin=candidate)
+        probe.step()  # the new list and plan, outside the timed run
+        seconds = _seconds(probe, steps)
+        longest = max(longest, _longest_row(probe))
+        tried.append((candidate, seconds, probe.number_of_neighbor_list_updates - before))
+        candidate += step_skin
+        if seconds > previous:
             break
-        old_time = t
+        previous = seconds
     if verbose:
-        print("Skin | Time per step (ms) | steps/updates")
-        for sk, t, u in zip(skins, times, updates):
-            print(f"{sk:4.1f} | {t / steps * 1000:6.4f}             | {steps}/{u} = {steps / max(u, 1):0.1f}")
-    best = times.index(min(times))
-    skin = skins[best]
+        print("skin   ms/step   rebuilds in the run")
+        for sk, seconds, rebuilds in tried:
+            print(f"{sk:5.2f}  {seconds / steps * 1e3:8.4f}  {rebuilds} in {steps} steps")
+    skin, best_seconds, _ = min(tried, key=lambda t: t[1])
     sim.set_neighbor_list(skin=skin)
 
-    # kernel shape at the chosen skin (no counterpart in the reference)
-    sim_copy = sim.copy()
-    sim_copy.step()
-    shapes = [(tpp, 0, fused) for tpp in (1, 2, 4, 8) for fused in (0, 1)]
-    shape_times = {}
-    for shp in shapes:
-        sim_copy.set_kernel_tuning(*shp)
-        sim_copy.step()
-        shape_times[shp] = _time_run(sim_copy, steps)
+    # -- kernel shape and cell width at that skin ----------------------------
+    probe = sim.copy()
+    probe.step()
+    timings = {}
+
+    def timed(tpp, fused, scale):
+        probe.set_kernel_tuning(tpp, 0, fused, scale)
+        probe.step()  # re-plan (and, for a new cell width, rebuild) outside the timed run
+        timings[(tpp, fused, scale)] = _seconds(probe, steps)
         if verbose:
-            print(f"threads/particle {shp[0]}, {'fused' if shp[2] else 'split'} step: "
-                  f"{shape_times[shp] / steps * 1000:.4f} ms per step")
-    best_shape = min(shape_times, key=shape_times.get)
-    sim.set_kernel_tuning(*best_shape)
+            print(f"{tpp} lanes per particle, {'fused' if fused else 'split'} step, cells "
+                  f"{scale:.2f} x cutoff: {timings[(tpp, fused, scale)] / steps * 1e3:.4f} ms per step")
+        return _longest_row(probe)
+
+    # shape first at the narrowest cells, then the cell width for that shape
+    for tpp in (1, 2, 4, 8):
+        for fused in (0, 1):
+            longest = max(longest, timed(tpp, fused, CELL_SCALES[0]))
+    tpp, fused, _ = min(timings, key=timings.get)
+    for scale in CELL_SCALES[1:]:
+        longest = max(longest, timed(tpp, fused, scale))
+    tpp, fused, scale = min(timings, key=timings.get)
+    sim.set_kernel_tuning(tpp, 0, fused, scale)
+
+    # -- row capacity -----------------------------------------------------
+    capacity = int(sim.max_number_of_neighbors)
+    if tune_capacity and longest > 0:
+        want = max(64, -(-int(1.5 * longest) // 8) * 8)
+        if want < capacity:
+            capacity = want
+            sim.set_neighbor_list(max_number_of_neighbors=capacity)
     if verbose:
-        print(f"Optimal parameters: skin={skin}, threads per particle={best_shape[0]}, "
-              f"{'fused' if best_shape[2] else 'split'} step")
+        print(f"chosen: skin {skin}, {tpp} lanes per particle, {'fused' if fused else 'split'} step, "
+              f"cells {scale:.2f} x cutoff, max_number_of_neighbors {capacity} (longest row seen {longest})")
     if plot:
         import matplotlib.pyplot as plt
-        plt.plot(skins, times, "o", label="Neighbour list")
-        plt.plot(skin, times[best], "ro", label="Fastest time")
-        plt.xlabel("Skin")
-        plt.ylabel("Time")
+        plt.plot([t[0] for t in tried], [t[1] for t in tried], "o", label="runs")
+        plt.plot(skin, best_seconds, "ro", label="fastest")
+        plt.xlabel("skin")
+        plt.ylabel("seconds")
         plt.legend()
         plt.show()
     return sim

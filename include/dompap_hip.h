@@ -150,6 +150,12 @@ int dp_integrate(dp_ctx* ctx, const double* forces, int integrator, const double
  * fused force+integrate kernel (1), force kernel then integrate kernel (0) or
  * automatic (-1). */
 int dp_set_tuning(dp_ctx* ctx, int threads_per_particle, int tile_particles, int fused);
+/* Cells at least scale x cutoff wide (1 <= scale <= 4; default 1): larger
+ * cells, fewer of them, larger tiles.  A tuning knob for the GPU autotuner
+ * (dompap/tools/autotune.py:4-178 has no counterpart); lists and forces do
+ * not depend on it beyond summation order.  The list is rebuilt at the next
+ * use. */
+int dp_set_cell_scale(dp_ctx* ctx, double scale);
 /* Current plan: tpp, entry bits, offset bits, fixed-point bits, LDS record
  * capacity (pair, emit), tiles, tile grid, fused, home / halo cells per tile
  * of replica 0, cells, steps per step block, whether timed step blocks record

@@ -1,6 +1,7 @@
 """The GPU autotuner keeps dompap's tools/autotune.py interface and the
 reference's check (tests/test_autotune.py:5-29: 0.1 < skin < 1.9); the tuned
-kernel shape gives the same forces (FP64 mode) as the planner's."""
+kernel shape, cell width and row capacity give the same forces (FP64 mode)
+as the planner's, per system shape."""
 import numpy as np
 import pytest
 
@@ -26,6 +27,36 @@ def test_autotune_reference_case():
     assert 0.1 < sim.neighbor_list_skin < 1.9, f"{sim.neighbor_list_skin=}"
     tpp, tile, fused = sim.__dict__["_tuning"]
     assert tpp in (1, 2, 4, 8) and fused in (0, 1)
+    assert sim.__dict__["_cell_scale"] in (1.0, 1.15, 1.3)
+    # row capacity: lowered from 512 to 1.5x the longest row, in multiples of 8
+    K = int(sim.max_number_of_neighbors)
     F1 = sim.get_forces()
+    longest = int(np.max(np.sum(np.asarray(sim.neighbor_list) >= 0, axis=1)))
+    assert K % 8 == 0 and 64 <= K < 512 and K >= longest
     assert np.abs(F1 - F0).max() <= 1e-12 * max(1.0, np.abs(F0).max())
     assert sim.get_kernel_plan()["threads_per_particle"] == tpp
+
+
+def test_autotune_lj_liquid_picks_shape_cells_and_capacity():
+    """A 3-D LJ liquid (N = 32,000): the tuned plan (lanes per particle,
+    cell width, capacity) runs and gives the planner's forces."""
+    from dompap_amd import Simulation
+    from dompap_amd.tools import autotune
+    sim = Simulation(seed=5)
+    fcc = ((0.0, 0.0, 0.0), (0.5, 0.5, 0.0), (0.5, 0.0, 0.5), (0.0, 0.5, 0.5))
+    sim.set_positions(unit_cell_coordinates=fcc, cells=(20, 20, 20))
+    sim.set_density(density=0.8442)
+    sim.set_masses(masses=1.0)
+    sim.set_random_velocities(temperature=0.72)
+    sim.set_pair_potential(pair_potential_str="4*(r**-12-r**-6)", r_cut=2.5)
+    sim.set_neighbor_list(skin=0.5, max_number_of_neighbors=512)
+    sim.set_integrator(time_step=0.005, target_temperature=0.72, temperature_damping_time=10.0)
+    sim.run(50)
+    F0 = sim.copy().get_forces()
+    sim = autotune(sim, steps=20, smallest_skin=0.3, step_skin=0.2)
+    F1 = sim.get_forces()
+    frms = float(np.sqrt(np.mean(np.sum(F0 ** 2, axis=1))))
+    assert np.abs(F1 - F0).max() / frms < 1e-12
+    plan = sim.get_kernel_plan()
+    assert plan["threads_per_particle"] == sim.__dict__["_tuning"][0]
+    assert 64 <= int(sim.max_number_of_neighbors) <= 512
