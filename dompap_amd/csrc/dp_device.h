@@ -59,6 +59,8 @@ struct DpRep {
   i32 collapsed_mask;   // bit d set if nc[d] == 1
   i32 order;            // 0: dimension 0 fastest, 1: dimension D-1 fastest
   i32 pot;              // potential of the replica (index into the context's pair tables)
+  i32 nslice;           // tiles per box of home cells: the home particles split into nslice
+                        // ranges, one workgroup each (single-cell replicas of many particles)
 };
 
 // Device control block (one per context).  The neighbour-list statistics are
@@ -296,6 +298,7 @@ struct DpTileSmem {
   i32 nrec;                    // staged records in the halo
   i32 maxc;                    // largest halo-cell occupancy
   i32 nhomep;                  // home particles
+  i32 k0, k1;                  // this tile's range of them (a slice; all of them when nslice == 1)
   i32 rep;                     // replica of the tile
   i32 fits;                    // 1 if the halo fits in the LDS record area
 };
@@ -400,7 +403,10 @@ __device__ void dp_tile_setup(DpTileSmem<D>& S, const DpRep* reps, int R, const 
   const int r = dp_tile_replica(reps, R, t);
   const DpRep& P = reps[r];
   int tc[D];
-  dp_unlinear<D>(t - P.t0, P.nt, P.order, tc);
+  const i64 lt = t - P.t0;
+  const i64 ct = lt / P.nslice;  // box of home cells; slice lt - ct * nslice of its particles
+  const int sl = (int)(lt - ct * P.nslice);
+  dp_unlinear<D>(ct, P.nt, P.order, tc);
   const int nh = P.nhalo;
   for (int u = tid; u < nh; u += blockDim.x) {
     int hc[D];
@@ -513,6 +519,9 @@ __device__ void dp_tile_setup(DpTileSmem<D>& S, const DpRep* reps, int R, const 
     if (lane == 0) {
       S.home_p[0] = 0;
       S.nhomep = carry;
+      const int h = (carry + P.nslice - 1) / P.nslice;
+      S.k0 = min(sl * h, carry);
+      S.k1 = min(S.k0 + h, carry);
     }
   }
   __syncthreads();
@@ -523,8 +532,9 @@ __device__ void dp_tile_setup(DpTileSmem<D>& S, const DpRep* reps, int R, const 
 // kernels with one coalesced copy instead of recomputing them:
 //   [0] nrec [1] nhomep [2] rep [3] maxc [4] nhalo [5] nhome
 //   [6] 1 if the halo exceeds the emit's LDS record area [7] unused,
+//   [8] [9] the tile's slice [k0, k1) of the home particles [10] [11] unused,
 //   then hstart[nh], hbase[nh + 1], hcoord[nh], home_u[nhm], home_p[nhm + 1].
-#define DP_TT_HEAD 8
+#define DP_TT_HEAD 12
 __host__ __device__ __forceinline__ int dp_tile_table_ints(int nh, int nhm) {
   return DP_TT_HEAD + 3 * nh + 1 + 2 * nhm + 1;
 }
@@ -540,6 +550,8 @@ __device__ void dp_tile_store(const DpTileSmem<D>& S, const DpRep& P, i32* tt) {
     tt[3] = S.maxc;
     tt[4] = nh;
     tt[5] = nhm;
+    tt[8] = S.k0;
+    tt[9] = S.k1;
   }
   i32* p = tt + DP_TT_HEAD;
   for (int u = tid; u < nh; u += blockDim.x) {
@@ -557,6 +569,7 @@ __device__ void dp_tile_load(DpTileSmem<D>& S, const i32* tt, int cap) {
   const int tid = threadIdx.x;
   const int4 h0 = *(const int4*)tt;
   const int4 h1 = *(const int4*)(tt + 4);
+  const int4 h2 = *(const int4*)(tt + 8);
   const int nh = h1.x, nhm = h1.y;
   const i32* p = tt + DP_TT_HEAD;
   for (int u = tid; u < nh; u += blockDim.x) {
@@ -569,6 +582,8 @@ __device__ void dp_tile_load(DpTileSmem<D>& S, const i32* tt, int cap) {
   if (tid == 0) {
     S.nrec = h0.x;
     S.nhomep = h0.y;
+    S.k0 = h2.x;
+    S.k1 = h2.y;
     S.rep = h0.z;
     S.maxc = h0.w;
     S.fits = h0.x <= cap ? 1 : 0;

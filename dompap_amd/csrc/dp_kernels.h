@@ -727,9 +727,13 @@ __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const 
   for (int m = 0; m < P.nhome; ++m) {
     const int uh = S.home_u[m];
     if (uh < 0) continue;
-    const int self0 = S.hbase[uh];
-    const int np = S.hbase[uh + 1] - self0;
-    const i64 slot0 = S.hstart[uh];
+    // this tile's slice of the cell's particles
+    const int ncell = S.hbase[uh + 1] - S.hbase[uh];
+    const int jlo = min(max(S.k0 - S.home_p[m], 0), ncell);
+    const int jhi = min(max(S.k1 - S.home_p[m], 0), ncell);
+    const int self0 = S.hbase[uh] + jlo;
+    const int np = jhi - jlo;
+    const i64 slot0 = S.hstart[uh] + jlo;
     // lane r < NROW holds stencil row r: its valid cells are consecutive halo cells
     int r_lo = 0, r_len = 0, r_u0 = 0;
     if (lane < NROW) {
@@ -1029,9 +1033,14 @@ __device__ __forceinline__ void dp_emit_particles_tab(const DpTileSmem<D>& S, co
     // (LDS values that are the same in every lane: read into scalar registers)
     const int uh = __builtin_amdgcn_readfirstlane(S.home_u[m]);
     if (uh < 0) continue;  // uniform over the workgroup: no barrier is skipped by part of it
-    const int self0 = __builtin_amdgcn_readfirstlane(S.hbase[uh]);
-    const int np = __builtin_amdgcn_readfirstlane(S.hbase[uh + 1]) - self0;
-    const i64 slot0 = __builtin_amdgcn_readfirstlane(S.hstart[uh]);
+    // this tile's slice of the cell's particles
+    const int ncell = __builtin_amdgcn_readfirstlane(S.hbase[uh + 1]) - __builtin_amdgcn_readfirstlane(S.hbase[uh]);
+    const int hp = __builtin_amdgcn_readfirstlane(S.home_p[m]);
+    const int jlo = min(max(S.k0 - hp, 0), ncell);
+    const int jhi = min(max(S.k1 - hp, 0), ncell);
+    const int self0 = __builtin_amdgcn_readfirstlane(S.hbase[uh]) + jlo;
+    const int np = jhi - jlo;
+    const i64 slot0 = __builtin_amdgcn_readfirstlane(S.hstart[uh]) + jlo;
     // lane r < NROW: stencil row r = cells 3r .. 3r + 2, consecutive halo cells
     int r_lo = 0, r_len = 0;
     if (lane < NROW) {
@@ -1096,7 +1105,7 @@ __device__ __forceinline__ void dp_emit_lanes(const DpTileSmem<D>& S, const DpRe
   constexpr int EPW = 64 / EB;              // entries per 8-byte store
   const i64 Ks = (i64)((K + 7) & ~7);
   const int lo16 = P.nlq16_lo, hi16 = P.nlq16_hi;
-  for (int k = threadIdx.x; k < S.nhomep; k += blockDim.x) {
+  for (int k = S.k0 + threadIdx.x; k < S.k1; k += blockDim.x) {
     const int m = dp_home_of<D>(S, P.nhome, k);
     const int uh = S.home_u[m];
     const int loc = k - S.home_p[m];
@@ -1331,7 +1340,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_nl_export(const E* nl, cons
   for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
     dp_tile_setup<D>(S, reps, R, cstart, t, 0x7fffffff);
     const DpRep& P = reps[__builtin_amdgcn_readfirstlane(S.rep)];  // uniform: scalar loads
-    for (int k = threadIdx.x; k < S.nhomep; k += blockDim.x) {
+    for (int k = S.k0 + threadIdx.x; k < S.k1; k += blockDim.x) {
       const int m = dp_home_of<D>(S, P.nhome, k);
       const int uh = S.home_u[m];
       const i64 slot_i = S.hstart[uh] + (k - S.home_p[m]);
@@ -1509,7 +1518,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_rdf_cells(const double* x, 
     dp_tile_load<D>(S, tiletab + t * (i64)tstride, 0);
     const DpRep& P = reps[__builtin_amdgcn_readfirstlane(S.rep)];  // uniform: scalar loads
     // a home particle meets at most every staged record of the halo
-    const unsigned long long tile_pairs = (unsigned long long)S.nhomep * (unsigned long long)S.nrec;
+    const unsigned long long tile_pairs = (unsigned long long)(S.k1 - S.k0) * (unsigned long long)S.nrec;
     if (pending + tile_pairs > DP_RDF_FLUSH_PAIRS) {
       dp_rdf_flush(hl, nb, hist);
       pending = 0;
@@ -1529,7 +1538,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_rdf_cells(const double* x, 
       S.st_off[st] = off;
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < S.nhomep; k += blockDim.x) {
+    for (int k = S.k0 + threadIdx.x; k < S.k1; k += blockDim.x) {
       const int m = dp_home_of<D>(S, P.nhome, k);
       const int uh = S.home_u[m];
       const i64 slot_i = S.hstart[uh] + (k - S.home_p[m]);

@@ -541,6 +541,17 @@ int refresh_reps(dp_ctx* ctx) {
         rmax = std::max(rmax, (std::isfinite(cut) ? cut : 0.0) + 2.0 * sk);
       }
     }
+    // a box of home cells holding many more particles than one pass of the
+    // workgroup's lane groups (a single-cell replica of hundreds of
+    // particles) is split into slices of its home particles, one workgroup
+    // each, so that it does not hold up a launch on one compute unit
+    {
+      const double exp_home = occ * (double)nhome;
+      P.nslice = 1;
+      if (exp_home > 2.0 * (double)target)
+        P.nslice = (i32)std::min<double>(std::ceil(exp_home / (double)target), 1024.0);
+      ntiles *= P.nslice;
+    }
     P.c0 = c0;
     P.ncell = ncell;
     P.t0 = t0;
@@ -656,6 +667,7 @@ int refresh_reps(dp_ctx* ctx) {
     cells_sig.push_back(P.c0);
     for (int d = 0; d < D; ++d) cells_sig.push_back(P.nc[d]);
     tiles_sig.push_back(P.t0);
+    tiles_sig.push_back(P.nslice);
     for (int d = 0; d < D; ++d) tiles_sig.push_back(P.T[d]);
   }
   tiles_sig.push_back(nlw);

@@ -368,9 +368,9 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
   const double* sxd = (const double*)stage;
   const i32* stype = (const i32*)(sxd + (size_t)a.cap * DP_D);
 #endif
-  for (int base = 0; base < S.nhomep; base += DP_GROUPS) {
+  for (int base = S.k0; base < S.k1; base += DP_GROUPS) {
     const int k = base + group;
-    const bool active = k < S.nhomep;
+    const bool active = k < S.k1;
     int uh = 0, loc = 0;
     if (active) {
       const int m = dp_home_of<DP_D>(S, P.nhome, k);

@@ -224,3 +224,40 @@ def test_256_replicas_against_c_oracle():
         assert abs(Eb[r] - o["energy"]) < 1e-6 * max(abs(o["energy"]), nr), r
         assert abs(Wb[r] - o["virial"]) < 1e-5 * max(abs(o["virial"]), nr), r
     b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_single_cell_box_split_into_slices(precision):
+    """A box narrower than three list cutoffs holds one cell; its 864
+    particles are split into slices of home particles, one workgroup each.
+    Lists (neighbor_list.py:7-39), forces, energy and virial as the oracle's,
+    before and after steps."""
+    from dompap_amd import Simulation
+    s = Simulation(seed=12, precision=precision)
+    s.set_positions(unit_cell_coordinates=FCC, cells=(6, 6, 6))
+    s.set_density(1.2)  # L = 8.96 < 3 x 3.0
+    s.set_masses(1.0)
+    rng = np.random.default_rng(12)
+    s.positions = np.asarray(s.positions) + rng.normal(0, 0.04, (864, 3))
+    s.velocities = rng.normal(0, 1.0, (864, 3))
+    s.set_pair_potential(LJ, r_cut=2.5)
+    s.set_neighbor_list(skin=0.5, max_number_of_neighbors=400)
+    s.set_integrator(time_step=0.002, target_temperature=1.0, temperature_damping_time=1.0)
+    plan = s.get_kernel_plan()
+    assert plan["cells"] == 1 and plan["tiles"] > 1, plan
+    u, f, c = R.make_pair_potential(LJ, 2.5)
+    box = np.asarray(s.box_vectors)
+    for k in range(2):
+        if k:
+            s.run(30)
+        x = np.asarray(s.positions)
+        xr = np.asarray(s.positions_neighbour_list)
+        assert np.array_equal(s.neighbor_list, R.neighbor_list_double_loop(xr, box, 3.0, 400))
+        nl = R.neighbor_list_double_loop(x, box, 3.0, 400)
+        Fo = R.forces(x, box, f, nl, ONE, ONE)
+        Eo = R.total_energy(x, box, u, nl, ONE, ONE)
+        Wo = R.virial_double_loop(x, box, f, ONE, ONE)
+        tol_f, tol_e = (1e-10, 1e-10) if precision == "fp64" else (1e-5, 1e-6)
+        assert np.abs(s.get_forces() - Fo).max() / _frms(Fo) < tol_f
+        assert abs(s.get_potential_energy() - Eo) < tol_e * abs(Eo)
+        assert abs(s.get_virial() - Wo) < tol_e * abs(Wo)
