@@ -763,7 +763,8 @@ __device__ void dp_tile_run(const DpPairArgs& a) {
   if (KIND == 2 || KIND == 3) dp_raise_rebuild(trip, dp_need_flag(a.ctl, a.blk_pos));
 }
 
-extern "C" __global__ void __launch_bounds__(DP_TILE_THREADS) dp_tile_force(DpPairArgs a) { dp_tile_run<0>(a); }
+// four workgroups (16 wavefronts) per compute unit: at most 128 VGPRs
+extern "C" __global__ void __launch_bounds__(DP_TILE_THREADS, 4) dp_tile_force(DpPairArgs a) { dp_tile_run<0>(a); }
 extern "C" __global__ void __launch_bounds__(DP_TILE_THREADS) dp_tile_obs(DpPairArgs a) { dp_tile_run<1>(a); }
 extern "C" __global__ void __launch_bounds__(DP_TILE_THREADS) dp_tile_gjf(DpPairArgs a) { dp_tile_run<2>(a); }
 extern "C" __global__ void __launch_bounds__(DP_TILE_THREADS) dp_tile_lf(DpPairArgs a) { dp_tile_run<3>(a); }
