@@ -523,8 +523,9 @@ def main():
     # The implementation's own bytes: entries of entry_bits, the staged
     # record (mixed: RS ints; FP64: D doubles), the row length, the force and
     # the share of the tile tables.  Integrate kernel: read x, v, beta, F,
-    # xref (5 D doubles), mass, cellpack, orig, rep; write x, v, beta (3 D
-    # doubles); mixed also rewrites its record (RS ints).
+    # xref (5 D doubles), cellpack, orig (and mass unless every mass is 1);
+    # write x, v, beta (3 D doubles); mixed also writes its record (RS ints,
+    # read first when the type slot must carry over).
     ent = plan["entry_bits"] // 8
     rs = 2
     while rs < dim + 1:
@@ -533,7 +534,9 @@ def main():
     tab_pp = (8 + 3 * plan["halo_cells_per_tile"] + 2 * plan["home_cells_per_tile"] + 2) * 4 / home
     rec = rs * 4 if args.precision == "mixed" else dim * 8
     b_force_impl = ent * kbar + rec + 4 + 8 * dim + tab_pp
-    b_integ = 8 * dim * 5 + 8 + 4 + 4 + 4 + 8 * dim * 3 + (2 * rs * 4 if args.precision == "mixed" else 0)
+    unit_mass = bool(np.all(wl["m"] == 1.0))
+    b_integ = (8 * dim * 5 + 4 + 4 + (0 if unit_mass else 8) + 8 * dim * 3
+               + ((rs * 4 * (2 if typed else 1)) if args.precision == "mixed" else 0))
     force_s = f_ms / 1e3 / nl_
     integ_s = i_ms / 1e3 / nl_
     achieved = n * b_force / force_s / 1e9

@@ -491,8 +491,16 @@ __global__ void k_cellsort(const i32* tmp, const i32* keys, const i32* cell_of, 
     const i32 c = cell_of[p];
     const i32 lo = cstart[c], hi = cstart[c + 1];
     const i32 key = keys[q];
+    // the cell's keys, four per 16-byte load where aligned (single-cell
+    // replicas hold hundreds of particles in one cell)
     i32 rnk = 0;
-    for (i32 k = lo; k < hi; ++k) rnk += (keys[k] < key) ? 1 : 0;
+    i32 k = lo;
+    for (; k < hi && (k & 3) != 0; ++k) rnk += (keys[k] < key) ? 1 : 0;
+    for (; k + 4 <= hi; k += 4) {
+      const int4 v = *(const int4*)(keys + k);
+      rnk += ((v.x < key) ? 1 : 0) + ((v.y < key) ? 1 : 0) + ((v.z < key) ? 1 : 0) + ((v.w < key) ? 1 : 0);
+    }
+    for (; k < hi; ++k) rnk += (keys[k] < key) ? 1 : 0;
     perm[lo + rnk] = p;
   }
 }
