@@ -163,7 +163,7 @@ def make_batch_engine(wl, precision, block_steps=100):
     return eng
 
 
-def make_engine(wl, precision, pair_precision="fp64", tuning=None):
+def make_engine(wl, precision, pair_precision="fp64", tuning=None, cell_scale=1.0):
     from dompap_amd.engine import Engine
     from dompap_amd.potential import device_potential
     n, dim = wl["x"].shape
@@ -178,6 +178,8 @@ def make_engine(wl, precision, pair_precision="fp64", tuning=None):
     eng.set_seed(wl["seed"], 0)
     if tuning is not None:
         eng.set_tuning(*tuning)
+    if cell_scale != 1.0:
+        eng.set_cell_scale(cell_scale)
     eng.set_field("positions", wl["x"])
     eng.set_field("velocities", wl["v"])
     eng.set_field("masses", wl["m"])
@@ -457,6 +459,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--tpp", type=int, default=0, help="lanes per particle (0: planner's choice)")
     ap.add_argument("--tile", type=int, default=0, help="home particles per tile (0: planner's choice)")
+    ap.add_argument("--cell-scale", type=float, default=1.0,
+                    help="cells at least this many list cutoffs wide (1.0: the narrowest)")
     ap.add_argument("--fused", type=int, default=-1, choices=[-1, 0, 1],
                     help="1: fused force + integrate kernel, 0: force kernel then integrate kernel, -1: planner")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
@@ -487,7 +491,7 @@ def main():
     tuning = None
     if args.tpp or args.tile or args.fused >= 0:
         tuning = (args.tpp, args.tile, args.fused)
-    eng = make_engine(wl, args.precision, args.pair_precision, tuning)
+    eng = make_engine(wl, args.precision, args.pair_precision, tuning, args.cell_scale)
     eng.step(args.warmup, "gjf")
     # every step-block graph the timed region replays (events recorded or
     # not, either buffer parity) is captured and instantiated here, outside

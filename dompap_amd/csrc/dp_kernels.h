@@ -949,10 +949,14 @@ template <int D, typename E> struct DpEmitCell {
 // table; returns the longest row.  Per chunk of 64 candidates and particle:
 // the 16-bit screen (lane masks of single compares, combined in scalar code),
 // then the hits' stores; a candidate in the screen's slack (rare) sends that
-// particle's chunk through the exact decision first.
+// particle's chunk through the exact decision first.  A hit's store address
+// is a 32-bit byte offset from the group's first row, clamped to the row's
+// last entry (a row longer than K is reported as an overflow; its entries are
+// not used).
 template <int D, typename E, int NJ>
 __device__ __forceinline__ int dp_emit_group(const DpEmitCell<D, E>& C, int k0) {
   constexpr int NW = DP_TILE_THREADS / 64;
+  constexpr int ESH = sizeof(E) == 2 ? 1 : 2;
   const int lane = threadIdx.x & 63;
   u32 pxy[NJ], pzw[NJ], cnt[NJ];
 #pragma unroll
@@ -962,8 +966,14 @@ __device__ __forceinline__ int dp_emit_group(const DpEmitCell<D, E>& C, int k0) 
     pzw[g] = __builtin_amdgcn_readfirstlane(dp_pack_hi16(r.z, r.w));
     cnt[g] = 0;
   }
-  const u32 row0 = (u32)((C.slot0 + k0) * C.Ks);  // nl index of the group's first row
-  const u32 rstep = (u32)(NW * C.Ks);
+  // the group's rows lie within NJ * NW rows of the first one
+  char* const rows = (char*)(C.nl + (C.slot0 + k0) * C.Ks);
+  const u32 rstepb = (u32)(NW * C.Ks) << ESH;
+  // the clamp bound in a vector register: scalar registers are the scarce
+  // resource of this loop (the bound would be reloaded from a spill slot)
+  u32 last;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(last) : "s"((u32)C.Ks - 1u));
+  const int self_c = C.cs0 + k0;  // candidate position of particle g's self entry: self_c + NW * g
   for (int base = 0; base < C.ncand; base += 64) {
     const int c = base + lane;
     const bool valid = c < C.ncand;
@@ -971,35 +981,28 @@ __device__ __forceinline__ int dp_emit_group(const DpEmitCell<D, E>& C, int k0) 
     const int4 qj = C.srq[idx];
     const u32 cxy = dp_pack_hi16(qj.x, qj.y), czw = dp_pack_hi16(qj.z, qj.w);
     const u64 vmask = __builtin_amdgcn_ballot_w64(valid);
-    // row offsets are recomputed per chunk in scalar code rather than held
-    // in registers across the loop
-    u32 rs = rstep;
-    asm volatile("" : "+s"(rs));
 #pragma unroll
     for (int g = 0; g < NJ; ++g) {
-      const int k = k0 + NW * g;
+      // lane masks from single compares, combined in scalar code
       const int s16 = dp_emit_s16<D>(pxy[g], pzw[g], cxy, czw);
-      const bool lt = s16 < C.lo16;
-      const bool ns = c != C.cs0 + k;
-      const u64 ltm = __builtin_amdgcn_ballot_w64(lt);
+      const u64 ltm = __builtin_amdgcn_ballot_w64(s16 < C.lo16);
       const u64 lem = __builtin_amdgcn_ballot_w64(s16 <= C.hi16);
-      const u32 rowg = row0 + (u32)g * rs;
-      if ((lem & ~ltm) == 0ull) {
-        // every candidate decided by the screen
-        const u64 hm = ltm & __builtin_amdgcn_ballot_w64(ns) & vmask;
-        const u32 pos = __builtin_amdgcn_mbcnt_hi((u32)(hm >> 32), __builtin_amdgcn_mbcnt_lo((u32)hm, cnt[g]));
-        if (valid && ns && lt && pos < (u32)C.K) C.nl[rowg + pos] = (E)idx;
-        cnt[g] += (u32)__popcll(hm);
-      } else {
+      const u64 okm = vmask & __builtin_amdgcn_ballot_w64(c != self_c + NW * g);
+      u64 hm = ltm & okm;
+      const u64 slack = lem & ~ltm & okm;
+      if (slack != 0ull) {
         // candidates in the screen's slack (rare): the exact decision
-        bool h = valid && ns && lt;
-        if (valid && ns && !lt && s16 <= C.hi16)
-          h = dp_emit_exact<D>(C.S, C.P, C.xref, C.srq, C.self0 + k, idx, C.slot0 + k);
-        const u64 hm = __builtin_amdgcn_ballot_w64(h);
-        const u32 pos = __builtin_amdgcn_mbcnt_hi((u32)(hm >> 32), __builtin_amdgcn_mbcnt_lo((u32)hm, cnt[g]));
-        if (h && pos < (u32)C.K) C.nl[rowg + pos] = (E)idx;
-        cnt[g] += (u32)__popcll(hm);
+        bool x = false;
+        if (__builtin_amdgcn_inverse_ballot_w64(slack))
+          x = dp_emit_exact<D>(C.S, C.P, C.xref, C.srq, C.self0 + k0 + NW * g, idx, C.slot0 + k0 + NW * g);
+        hm |= __builtin_amdgcn_ballot_w64(x);
       }
+      if (__builtin_amdgcn_inverse_ballot_w64(hm)) {
+        const u32 e = min(__builtin_amdgcn_mbcnt_hi((u32)(hm >> 32), __builtin_amdgcn_mbcnt_lo((u32)hm, 0u)) + cnt[g],
+                          last);
+        *(E*)(rows + (u64)((e << ESH) + (u32)g * rstepb)) = (E)idx;
+      }
+      cnt[g] += (u32)__popcll(hm);
     }
   }
   int mx = 0;
