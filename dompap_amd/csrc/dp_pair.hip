@@ -74,6 +74,15 @@ __device__ __forceinline__ T dp_group_sum(T v) {
   return v;
 }
 
+// The same over a group of tpp lanes (a power of two, the same in the whole
+// wavefront): DP_TPP lanes, or more for a tile's last few particles.
+template <class T>
+__device__ __forceinline__ T dp_group_sum(T v, int tpp) {
+  if (tpp == DP_TPP) return dp_group_sum(v);
+  for (int off = tpp >> 1; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
 // Neighbour rows: stride K rounded up to 8 entries, so that every lane reads
 // its entries 8 bytes at a time (4 16-bit or 2 32-bit entries).
 __device__ __forceinline__ i64 dp_row_stride(int K) { return (i64)((K + 7) & ~7); }
@@ -105,13 +114,13 @@ __device__ __forceinline__ DpLoad dp_ld_entries(const dp_ent_t* p) {
 }
 
 // Call fn(idx, nvalid) for the chunks of a row of length cnt that belong to
-// lane `sub` of its group: loads of DP_CPL chunks at entries
-// [W * (sub + j * TPP), + W), W = DP_EPL * DP_CPL, for j = 0, 1, ...; idx[c]
+// lane `sub` of its group of tpp lanes: loads of DP_CPL chunks at entries
+// [W * (sub + j * tpp), + W), W = DP_EPL * DP_CPL, for j = 0, 1, ...; idx[c]
 // for c >= nvalid is unspecified.
 template <class Fn>
-__device__ __forceinline__ void dp_for_chunks(const dp_ent_t* row, int cnt, int sub, Fn& fn) {
+__device__ __forceinline__ void dp_for_chunks(const dp_ent_t* row, int cnt, int sub, int tpp, Fn& fn) {
   constexpr int W = DP_EPL * DP_CPL;
-  const int step = W * DP_TPP;
+  const int step = W * tpp;
   int k0 = sub * W;
   DpLoad buf[DP_PF];
 #pragma unroll
@@ -358,8 +367,6 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
   (void)spc;
   DP_PC_DECL
   const int tid = threadIdx.x;
-  const int group = tid / DP_TPP;
-  const int sub = tid % DP_TPP;
   const dp_ent_t* nl = (const dp_ent_t*)a.nl;
   const int nh = P.nhalo;
 #if DP_MODE == 1
@@ -369,8 +376,30 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
   const i32* stype = (const i32*)(sxd + (size_t)a.cap * DP_D);
 #endif
   for (int base = S.k0; base < S.k1; base += DP_GROUPS) {
+    // lanes: DP_TPP per particle; in a pass of fewer than DP_GROUPS
+    // particles, the particles beyond the wavefronts that it fills take the
+    // remaining wavefronts with tpp lanes each (a power of two, >= DP_D, the
+    // same in the whole wavefront), so that a wavefront holding a few
+    // particles does not walk their rows DP_TPP lanes at a time
+    int tpp = DP_TPP, group = tid / DP_TPP, sub = tid % DP_TPP;
+    const int np = min(S.k1 - base, DP_GROUPS);
+    if (np < DP_GROUPS) {
+      constexpr int PW = 64 / DP_TPP;  // particles per wavefront at DP_TPP lanes
+      const int wf = np / PW;           // wavefronts filled at DP_TPP lanes
+      const int nt = np - wf * PW;      // particles left
+      const int nw = DP_TILE_THREADS / 64 - wf;
+      const int wave = tid >> 6;
+      int t = DP_TPP;
+      while (t < 64 && nt * 2 * t <= 64 * nw) t *= 2;
+      if (wave >= wf && nt > 0 && t >= DP_D && t > DP_TPP) {
+        const int lt = tid - wf * 64;
+        tpp = t;
+        group = wf * PW + lt / t;
+        sub = lt % t;
+      }
+    }
     const int k = base + group;
-    const bool active = k < S.k1;
+    const bool active = group < np;
     int uh = 0, loc = 0;
     if (active) {
       const int m = dp_home_of<DP_D>(S, P.nhome, k);
@@ -503,7 +532,7 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
 #endif
       }
     };
-    dp_for_chunks(row, cnt, sub, chunk);
+    dp_for_chunks(row, cnt, sub, tpp, chunk);
 #else
     // ---------------- FP64 parity mode ----------------
     double xi_[DP_D];
@@ -556,23 +585,23 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
         }
       }
     };
-    dp_for_chunks(row, cnt, sub, chunk);
+    dp_for_chunks(row, cnt, sub, tpp, chunk);
 #endif
     // combine the partial sums of the particle's DP_TPP lanes (every lane
     // of the group ends with the totals)
 #pragma unroll
-    for (int d = 0; d < DP_D; ++d) f[d] = dp_group_sum(f[d]);
+    for (int d = 0; d < DP_D; ++d) f[d] = dp_group_sum(f[d], tpp);
     if (KIND == 1) {
-      e = dp_group_sum(e);
-      w = dp_group_sum(w);
-      lap = dp_group_sum(lap);
+      e = dp_group_sum(e, tpp);
+      w = dp_group_sum(w, tpp);
+      lap = dp_group_sum(lap, tpp);
     }
     if (!active) continue;
     if (KIND == 0 || KIND == 1) {
       if (a.F) {
 #pragma unroll
         for (int d = 0; d < DP_D; ++d)
-          if (d % DP_TPP == sub) a.F[slot_i * DP_D + d] = f[d];
+          if (d % tpp == sub) a.F[slot_i * DP_D + d] = f[d];
       }
       if (KIND == 1 && sub == 0) {
         if (a.E) a.E[slot_i] = e;
@@ -582,8 +611,10 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
       continue;
     }
     // fused integrator update of particle slot_i: lane `sub` of the group
-    // updates dimensions sub, sub + DP_TPP, ...; the normal numbers are drawn
-    // in one SIMD pass when DP_TPP >= DP_D
+    // updates dimensions sub, sub + tpp, ...; the normal numbers are drawn
+    // in one SIMD pass when tpp >= DP_D.  A lane's j-th dimension is held at
+    // index j: d / DP_TPP, or 0 where tpp > DP_TPP (then tpp >= DP_D: one
+    // dimension per lane)
     {
 #pragma clang fp contract(off)
       const i64 ii = slot_i;
@@ -594,7 +625,7 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
         const u32 pid = (u32)a.orig[ii];
 #pragma unroll
         for (int j = 0; j < DP_JMAX; ++j) {
-          const int d = sub + j * DP_TPP;
+          const int d = sub + j * tpp;
           z_own[j] = 0.0;
           // device noise: keyed by the replica's seed and the particle's id in its replica
           if (d < DP_D)
@@ -612,10 +643,11 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
       for (int j = 0; j < DP_JMAX; ++j) tq_own[j] = 0.0;
 #pragma unroll
       for (int d = 0; d < DP_D; ++d) {
-        if (d % DP_TPP != sub) continue;
+        if (d % tpp != sub) continue;
+        const int jd = (tpp == DP_TPP) ? d / DP_TPP : 0;
         double xn;
         if (KIND == 2) {
-          const double nb = sq * z_own[d / DP_TPP];
+          const double nb = sq * z_own[jd];
           const double v1 = dp_gjf_v(ca, cb, dt, m, a.v[ii * DP_D + d], f[d], a.beta[ii * DP_D + d], nb);
           xn = a.x[ii * DP_D + d] + v1 * dt;
           a.v[ii * DP_D + d] = v1;
@@ -635,7 +667,7 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
           }
         }
         a.x_out[ii * DP_D + d] = xn;
-        tq_own[d / DP_TPP] = dp_minimage_gt(dp_sub(xn, a.xref[ii * DP_D + d]), P.L[d], P.halfL[d]);
+        tq_own[jd] = dp_minimage_gt(dp_sub(xn, a.xref[ii * DP_D + d]), P.L[d], P.halfL[d]);
 #if DP_MODE == 1
         const u32 c = (pack >> (DP_PACK_BITS * d)) & ((1u << DP_PACK_BITS) - 1u);
         const double cen = ((double)c + 0.5) * P.w[d];
@@ -650,11 +682,12 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
       }
 #endif
       // squared displacement summed in dimension order from the owners
-      const int gbase = (tid & 63) & ~(DP_TPP - 1);
+      const int gbase = (tid & 63) & ~(tpp - 1);
       double sdisp = 0.0;
 #pragma unroll
       for (int d = 0; d < DP_D; ++d) {
-        const double tq = (DP_TPP == 1) ? tq_own[d] : __shfl(tq_own[d / DP_TPP], gbase + (d % DP_TPP));
+        const int jd = (tpp == DP_TPP) ? d / DP_TPP : 0;
+        const double tq = (tpp == 1) ? tq_own[d] : __shfl(tq_own[jd], gbase + (d % tpp));
         sdisp = (d == 0) ? dp_mul(tq, tq) : dp_add(sdisp, dp_mul(tq, tq));
       }
       trip |= dp_skin_trips(sdisp, P.skin_half_sq);
