@@ -142,7 +142,7 @@ def replica_workload(seed, n_rep=256):
                 ns=ns, pot="4*(r**-12-r**-6)", r_cut=2.5, skin=0.5, dt=0.005, tau=10.0, K=256, seed=seed)
 
 
-def make_batch_engine(wl, precision, block_steps=100):
+def make_batch_engine(wl, precision, block_steps=100, tuning=None):
     from dompap_amd.engine import Engine
     from dompap_amd.potential import device_potential
     n = wl["x"].shape[0]
@@ -156,6 +156,8 @@ def make_batch_engine(wl, precision, block_steps=100):
     eng.set_seed(wl["seed"], 0)
     eng.set_replica_seeds(np.arange(R, dtype=np.uint64) + np.uint64(wl["seed"]))
     eng.set_graph_block(block_steps)
+    if tuning is not None:
+        eng.set_tuning(*tuning)
     eng.set_field("positions", wl["x"])
     eng.set_field("velocities", wl["v"])
     eng.set_field("masses", wl["m"])
@@ -335,7 +337,10 @@ def run_replicas(args, log):
     n = wl["x"].shape[0]
     R = len(wl["ns"])
     log(f"workload {wl['desc']} precision={args.precision}")
-    eng = make_batch_engine(wl, args.precision)
+    tuning = None
+    if args.tpp or args.tile or args.fused >= 0:
+        tuning = (args.tpp, args.tile, args.fused)
+    eng = make_batch_engine(wl, args.precision, tuning=tuning)
     eng.step(args.warmup, "gjf")
     eng.prepare_step_graphs("gjf", timed=True)
     eng.prepare_step_graphs("gjf", timed=False)

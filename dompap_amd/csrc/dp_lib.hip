@@ -243,6 +243,7 @@ struct DpEv3 {
 
 struct dp_ctx {
   int dev = 0;
+  int ncu = 256;  // compute units of the device
   hipStream_t st = nullptr;
   int D = 3, mode = DP_FP64, R = 1, rs = 4;
   i64 N = 0;
@@ -406,8 +407,9 @@ double ball_volume(int D) {
 // boxes, cutoffs and particle counts.
 //  - cells: nc = floor(L / cutoff) per dimension (1 if fewer than 3 fit);
 //  - lanes per particle (tpp) from the expected list length;
-//  - tiles: enough home cells that one pass of the workgroup's lane groups
-//    is about full;
+//  - tiles: enough home cells that two passes of the workgroup's lane groups
+//    are about full (one pass for systems too small to give 8 such tiles per
+//    compute unit);
 //  - fixed-point scale: coordinates relative to a tile's first home cell
 //    centre stay below 2^30 units;
 //  - entries: index of the neighbour in its tile's staged halo, 16-bit when
@@ -442,8 +444,15 @@ int refresh_reps(dp_ctx* ctx) {
   else if (kbar_max >= 24) tpp = 2;
   if (ctx->tune_tpp > 0) tpp = ctx->tune_tpp;
   ctx->fused = ctx->tune_fused >= 0 ? ctx->tune_fused : 0;
-  // home particles per tile: one pass of the workgroup's lane groups
-  const int target = ctx->tune_tile > 0 ? ctx->tune_tile : DP_TILE_THREADS / tpp;
+  // home particles per tile: two passes of the workgroup's lane groups
+  // (fewer tiles stage fewer halo records per particle; the last pass's
+  // leftover particles take the idle wavefronts with more lanes each), or
+  // one pass where two would leave fewer than 8 tiles per compute unit
+  // (measured on MI355X: 3-D LJ at N = 2^20 +6.6 %, 2-D r^-12 at N = 2^22
+  // +8 %; 3-D LJ at N = 4,096 -13 % with two passes)
+  const int pass = DP_TILE_THREADS / tpp;
+  const bool two = (double)ctx->N / (2.0 * pass) >= 8.0 * ctx->ncu;
+  const int target = ctx->tune_tile > 0 ? ctx->tune_tile : (two ? 2 * pass : pass);
   // LDS record capacity of the pair kernels (48 KB of records)
   // staged record: mixed, the coordinates (+ the type with several types) in
   // a power of two of ints (DP_LS of the pair kernels); FP64, D doubles (+ the
@@ -511,6 +520,10 @@ int refresh_reps(dp_ctx* ctx) {
       if (occ * (double)nT > (double)target || nT > DP_HOME_MAX || nH > DP_HALO_MAX ||
           occ * (double)nH + 32.0 > (double)lds_rec_max)
         break;
+      // beyond one pass only while the halo stays well inside the LDS record
+      // area (dense replicas of a batch would otherwise size the area for
+      // every tile: 256 mixed replicas -3.5 % with their 8-cell tiles)
+      if (occ * (double)nT > (double)pass && occ * (double)nH + 32.0 > 0.6 * (double)lds_rec_max) break;
       T[best] += 1;
     }
     i64 ntiles = 1, nhome = 1, nhalo = 1;
@@ -548,8 +561,8 @@ int refresh_reps(dp_ctx* ctx) {
     {
       const double exp_home = occ * (double)nhome;
       P.nslice = 1;
-      if (exp_home > 2.0 * (double)target)
-        P.nslice = (i32)std::min<double>(std::ceil(exp_home / (double)target), 1024.0);
+      if (exp_home > 2.0 * (double)pass)
+        P.nslice = (i32)std::min<double>(std::ceil(exp_home / (double)pass), 1024.0);
       ntiles *= P.nslice;
     }
     P.c0 = c0;
@@ -1421,6 +1434,11 @@ DP_API int dp_create(dp_ctx** out, int device, int dim, int64_t n_total, int n_r
   if (hipSetDevice(device) != hipSuccess) return bail(ctx->fail(DP_ERR_HIP, "hipSetDevice failed"));
   if (hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess)
     return bail(ctx->fail(DP_ERR_HIP, "hipStreamCreate failed"));
+  {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+      ctx->ncu = ncu;
+  }
   {
     // device address of the allocation-free entry point's scratch, once per device
     std::string serr;
