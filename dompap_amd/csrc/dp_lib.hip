@@ -695,7 +695,10 @@ int refresh_reps(dp_ctx* ctx) {
   ctx->cap = (int)cap;
   ctx->lds_pair = (size_t)cap * rec_bytes;
   {
-    i64 ce = (i64)std::ceil(1.5 * halo_occ_max) + 64;
+    // the emit's record area by the same rule (mean halo plus three Poisson
+    // standard deviations): with 8-cell tiles four emit workgroups fit a
+    // compute unit instead of three
+    i64 ce = (i64)std::ceil(std::min(1.5 * halo_occ_max, halo_occ_max + 3.0 * std::sqrt(halo_occ_max))) + 64;
     ce = std::min<i64>(ce, (i64)(48 * 1024 / 16));
     ctx->cap_emit = (int)std::max<i64>(ce, 64);
   }
