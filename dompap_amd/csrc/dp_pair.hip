@@ -729,10 +729,54 @@ __device__ void dp_tile_run(const DpPairArgs& a) {
     const DpRep& P = a.reps[__builtin_amdgcn_readfirstlane(S.rep)];  // uniform: scalar loads
     const int nh = P.nhalo;
     const bool fits = S.fits != 0;
+    // rows of the halo: its cells along the fastest dimension (consecutive
+    // halo indices, so consecutive staged records and, within each cell,
+    // consecutive slots)
+    const int rl = P.Th[P.order == 0 ? 0 : DP_D - 1];
+    const int nrow = nh / rl;
+#if DP_MODE == 1 && (DP_D == 2 || DP_D == 3)
     if (fits) {
-#if DP_MODE == 1
-      // DP_SB records per thread at a time: their loads are all issued before
-      // the first one is waited for
+      // row by row: wavefront w takes rows w, w + NW, ..; lane l copies
+      // records l, l + 64, .. of each (its halo cell from the row's few cell
+      // boundaries, no search over the halo), two loads issued before the
+      // first one is waited for
+      constexpr int NW = DP_TILE_THREADS / 64;
+      const int lane = tid & 63;
+      const int wave = tid >> 6;
+      for (int row = wave; row < nrow; row += NW) {
+        const int u0 = row * rl;
+        const int b0 = S.hbase[u0], b1 = S.hbase[u0 + rl];
+        for (int i0 = b0 + lane; i0 < b1; i0 += 128) {
+          DpRec r[2];
+          int cu[2];  // halo cell of the record (-1: none)
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const int i = i0 + 64 * q;
+            cu[q] = -1;
+            if (i < b1) {
+              int u = u0;
+              for (int k = 1; k < rl; ++k) u += (i >= S.hbase[u0 + k]) ? 1 : 0;
+              r[q] = dp_ld_lrec(a.xq + ((i64)S.hstart[u] + (i - S.hbase[u])) * DP_RS);
+              cu[q] = u;
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            if (cu[q] >= 0) {
+#pragma unroll
+              for (int d = 0; d < DP_D; ++d)
+                r[q].v[d] = (i32)((u32)r[q].v[d] + dp_halo_shift<DP_D>(S, P, cu[q], d));
+              dp_st_lrec(stage_i + (i0 + 64 * q) * DP_LS, r[q]);
+            }
+          }
+        }
+      }
+    }
+#elif DP_MODE == 1
+    if (fits) {
+      // 1-D (one long row) and 4-D (rows of a few records): DP_SB records per
+      // thread at a time, their loads all issued before the first one is
+      // waited for
       for (int b0 = tid; b0 < S.nrec; b0 += DP_SB * DP_TILE_THREADS) {
         DpRec r[DP_SB];
         u32 sh[DP_SB][DP_D];
@@ -756,7 +800,9 @@ __device__ void dp_tile_run(const DpPairArgs& a) {
           }
         }
       }
+    }
 #else
+    if (fits) {
       double* sxd = (double*)stage_i;
       i32* stype = (i32*)(sxd + (size_t)a.cap * DP_D);
       for (int idx = tid; idx < S.nrec; idx += blockDim.x) {
@@ -767,8 +813,8 @@ __device__ void dp_tile_run(const DpPairArgs& a) {
         for (int d = 0; d < DP_D; ++d) sxd[idx * DP_D + d] = t[d];
         if (DP_NTYPES > 1) stype[idx] = a.type[slot];
       }
-#endif
     }
+#endif
 #if DP_NTYPES > 1
     DP_POT_CALL(P.pot, dp_fill_spc(spc, a.tab[P.pot]));
 #endif
