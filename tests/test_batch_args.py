@@ -9,9 +9,13 @@ def test_batch_rejects_inconsistent_replicas():
     b.set_positions(([0.0, 0.0],), (5, 5), (1.0, 1.0))
     with pytest.raises(ValueError, match="dimension"):
         ReplicaBatch([a, b])
-    c = Simulation()
-    c.set_pair_potential("4*(r**-12-r**-6)", r_cut=2.5)
-    with pytest.raises(ValueError, match="pair potential"):
-        ReplicaBatch([a, c])
+    # replicas may differ in potential, but a batch holds at most 8 distinct ones
+    many = []
+    for k in range(9):
+        s = Simulation()
+        s.set_pair_potential("4*(r**-12-r**-6)", r_cut=2.0 + 0.1 * k)
+        many.append(s)
+    with pytest.raises(ValueError, match="at most 8 distinct potentials"):
+        ReplicaBatch(many)
     with pytest.raises(ValueError, match="at least one"):
         ReplicaBatch([])
