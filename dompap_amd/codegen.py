@@ -13,7 +13,10 @@ Generated functions (x = r / sigma):
 * ``dp_pot_gs<T>(s)`` f(x)/x as a function of s = x^2, and
   ``dp_pot_us<T>(s)`` u(x) as a function of s, when both are expressible
   without odd powers of x ("even" potentials such as Lennard-Jones or r^-n):
-  the mixed kernel then needs neither sqrt nor a division per pair.
+  the mixed kernel then needs neither sqrt nor a division per pair;
+* ``dp_pot_gsk<T>(s, a)`` (f(x)/x for a polynomial in 1/s of at most four
+  terms) with runtime coefficients into which the host folds the pair
+  kernel's scales, and its ``// dp-gsk:`` line of exponents and coefficients.
 """
 from __future__ import annotations
 
@@ -233,6 +236,46 @@ def _emit_scaled(name: str, expr: sp.Expr, var: sp.Symbol):
     return _emit(name, expr, var), sp.Integer(1)
 
 
+def _emit_folded(name: str, expr: sp.Expr, var: sp.Symbol):
+    """For ``expr`` a polynomial in q = 1/var, the device function
+    ``name<T>(var, a)`` = sum_k a[k] * z^e_k with z = 1/var, evaluated as
+    z^e_0 (a[0] + z^(e_1 - e_0) (a[1] + ...)), and the comment line
+    ``// dp-gsk: n e_0 c_0 e_1 c_1 ...`` giving the exponents e_k and the
+    coefficients c_k of expr = sum_k c_k var^-e_k.  The host folds its
+    per-type-pair scales into a[k] = ke * c_k * ks^-e_k (ks: the scale of
+    var, ke: the prefactor), so that the kernel evaluates the scaled term
+    with neither the scale multiply nor the prefactor multiply.  None when
+    expr is not such a polynomial or has more than 4 terms."""
+    expr = sp.sympify(expr)
+    q = sp.Symbol("dp_q", positive=True)
+    inv = sp.expand(expr.subs(var, 1 / q))
+    if not (expr.free_symbols == {var} and inv.is_polynomial(q) and not expr.is_polynomial(var)):
+        return None
+    content, prim = sp.Poly(inv, q).primitive()
+    if content == 0:
+        return None
+    if prim.LC() < 0:
+        prim = -prim
+    terms = sorted((int(m[0]), sp.Rational(c)) for m, c in prim.terms() if c != 0)
+    sign = 1 if sp.Poly(inv, q).LC() * prim.LC() > 0 else -1
+    if not terms or len(terms) > 4:
+        return None
+    z = "dp_z"
+    inner = f"a[{len(terms) - 1}]"
+    for k in range(len(terms) - 2, -1, -1):
+        de = terms[k + 1][0] - terms[k][0]
+        inner = f"(a[{k}] + dp_ipow<{de}, T>({z}) * {inner})"
+    e0 = terms[0][0]
+    body = inner if e0 == 0 else f"dp_ipow<{e0}, T>({z}) * {inner}"
+    # the coefficients of expr itself (content times primitive part)
+    meta = " ".join(f"{e} {float(sign * content * c)!r}" for e, c in terms)
+    src = (f"#define DP_GSK_N {len(terms)}\n"
+           f"// dp-gsk: {len(terms)} {meta}\n"
+           f"template <class T> __device__ __forceinline__ T {name}(T {var.name}, const T* a) {{\n"
+           f"  const T {z} = dp_rcp({var.name});\n  return {body};\n}}\n")
+    return src
+
+
 def _is_even_form(expr_s: sp.Expr, s: sp.Symbol) -> bool:
     for p in sp.preorder_traversal(expr_s):
         if isinstance(p, sp.Pow) and p.base.has(s) and not p.exp.is_Integer:
@@ -282,4 +325,8 @@ def generate(pair_potential_str: str) -> PotentialCode:
         # kernels' prefactor)
         src, scale = _emit_scaled("dp_pot_gsn", g_s, s)
         parts.append(f"#define DP_GS_SCALE ({float(scale)!r})\n" + src)
+        # the same term with the scales folded into its coefficients
+        folded = _emit_folded("dp_pot_gsk", g_s, s)
+        if folded is not None:
+            parts.append(folded)
     return PotentialCode(source="\n".join(parts), even=even, expression=pair_potential_str)

@@ -123,6 +123,13 @@ struct DpPairTab {
   double qscale;        // 2^-qbits
   double qscale2;       // 2^-2qbits
   double qinv;          // 2^qbits
+  // mixed mode, potentials whose f(x)/x is a polynomial in 1/x^2 (at most
+  // four terms c_k x^-2e_k): per type pair, the coefficients
+  // a_k = c_k * eps / sigma * 2^-q * (sigma^2 2^2q)^e_k of the force term
+  // per fixed-point unit as a polynomial in 1/s, s the squared fixed-point
+  // distance; `fold` = 1 when every a_k is a normal number
+  double gk[DP_MAXTYPES * DP_MAXTYPES * 4];
+  i32 fold;
 };
 
 // Arguments of the hiprtc-built pair / step kernels (passed by value).

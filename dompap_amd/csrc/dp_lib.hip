@@ -157,7 +157,8 @@ void split_pair_source(std::string& pre, std::string& pot, std::string& post) {
 // The pair kernels' source for potentials srcs[k] (generated device
 // functions; evens[k] = 1 when they take s = x^2), replica r using potential
 // DpRep::pot.
-std::string pair_source(const PairConfig& c, const std::vector<std::string>& srcs, const std::vector<int>& evens) {
+std::string pair_source(const PairConfig& c, const std::vector<std::string>& srcs, const std::vector<int>& evens,
+                        const std::vector<int>& folds) {
   std::string src;
   char buf[512];
   snprintf(buf, sizeof(buf),
@@ -175,12 +176,14 @@ std::string pair_source(const PairConfig& c, const std::vector<std::string>& src
   src += pre;
   std::string call = "#define DP_POT_CALL(pot, ...) switch (pot) {";
   for (size_t k = 0; k < srcs.size(); ++k) {
-    snprintf(buf, sizeof(buf), "namespace dp_p%d {\n#define DP_EVEN %d\n", (int)k, evens[k] ? 1 : 0);
+    const int fold = (k < folds.size()) ? folds[k] : 0;
+    snprintf(buf, sizeof(buf), "namespace dp_p%d {\n#define DP_EVEN %d\n#define DP_FOLD %d\n", (int)k,
+             evens[k] ? 1 : 0, fold);
     src += buf;
     src += srcs[k];
     src += "\n";
     src += pot;
-    src += "#undef DP_EVEN\n}\n";
+    src += "#undef DP_EVEN\n#undef DP_FOLD\n}\n";
     snprintf(buf, sizeof(buf), " case %d: { using namespace dp_p%d; __VA_ARGS__; } break;", (int)k, (int)k);
     call += buf;
   }
@@ -190,8 +193,32 @@ std::string pair_source(const PairConfig& c, const std::vector<std::string>& src
   return src;
 }
 
-std::string pair_source(const PairConfig& c, const char* device_src) {
-  return pair_source(c, std::vector<std::string>{device_src}, std::vector<int>{c.even});
+std::string pair_source(const PairConfig& c, const char* device_src, int fold) {
+  return pair_source(c, std::vector<std::string>{device_src}, std::vector<int>{c.even}, std::vector<int>{fold});
+}
+
+// The "// dp-gsk: n e_0 c_0 .." line of a generated potential source: f(x)/x
+// = sum_k c_k s^-e_k with s = x^2 (at most four terms); false if absent.
+bool parse_gsk(const std::string& src, int& n, int* e, double* cf) {
+  const size_t at = src.find("// dp-gsk: ");
+  if (at == std::string::npos) return false;
+  const char* p = src.c_str() + at + 11;
+  char* end = nullptr;
+  const long nn = std::strtol(p, &end, 10);
+  if (end == p || nn < 1 || nn > 4) return false;
+  p = end;
+  for (int k = 0; k < (int)nn; ++k) {
+    const long ek = std::strtol(p, &end, 10);
+    if (end == p || ek < 0 || ek > 64) return false;
+    p = end;
+    const double ck = std::strtod(p, &end);
+    if (end == p || !std::isfinite(ck)) return false;
+    p = end;
+    e[k] = (int)ek;
+    cf[k] = ck;
+  }
+  n = (int)nn;
+  return true;
 }
 
 // hiprtc compile for gfx950 (no device needed), memoised per process.
@@ -334,6 +361,8 @@ struct dp_ctx {
 
   // kernel timing
   bool timing = false;
+  int timing_every = 1;   // step blocks: events in one block of every timing_every
+  int64_t timing_block = 0;  // step blocks launched since timing was enabled
   std::vector<DpEv3> ev_pending;
   std::vector<DpEv3> ev_free;
   double time_ms = 0.0, time_force_ms = 0.0, time_integ_ms = 0.0;
@@ -833,6 +862,32 @@ int refresh_tab(dp_ctx* ctx) {
         T.guard_hi_q[k] = hi;
       }
   }
+  // folded force coefficients: a_j = c_j * eps / sigma * sigma^(2 e_j) *
+  // 2^((2 e_j - 1) q) per type pair (extended precision, then one rounding);
+  // folding is off for a potential whose coefficients leave [2^-1000, 2^1000]
+  for (size_t p = 0; p < ctx->tabs_h.size(); ++p) {
+    DpPairTab& T = ctx->tabs_h[p];
+    T.fold = 0;
+    int n = 0, ex[4];
+    double cf[4];
+    if (ctx->mode != DP_MIXED || ctx->pair_prec != 64 || p >= ctx->pot_srcs.size() || !ctx->pot_evens[p] ||
+        !parse_gsk(ctx->pot_srcs[p], n, ex, cf))
+      continue;
+    bool ok = true;
+    for (int k = 0; k < nt * nt; ++k) {
+      const long double sg = T.sigma[k], ep = T.eps[k];
+      for (int j = 0; j < 4; ++j) {
+        double a = 0.0;
+        if (j < n) {
+          const long double v = (long double)cf[j] * ep / sg * std::pow(sg * sg, (long double)ex[j]);
+          a = (double)std::ldexp(v, (2 * ex[j] - 1) * ctx->qbits);
+          if (!std::isfinite(a) || (a != 0.0 && std::fabs(std::log2(std::fabs(a))) > 1000.0)) ok = false;
+        }
+        T.gk[k * 4 + j] = a;
+      }
+    }
+    T.fold = ok ? 1 : 0;
+  }
   HIPCHK(hipMemcpyAsync(ctx->tab_d, ctx->tabs_h.data(), sizeof(DpPairTab) * ctx->tabs_h.size(),
                         hipMemcpyHostToDevice, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
@@ -852,7 +907,9 @@ int ensure_module(dp_ctx* ctx) {
   }
   c.hmax = std::min((hmax + 15) / 16 * 16, DP_HALO_MAX);
   c.omax = std::min((omax + 15) / 16 * 16, DP_HOME_MAX);
-  std::string src = pair_source(c, ctx->pot_srcs, ctx->pot_evens);
+  std::vector<int> folds;
+  for (const DpPairTab& T : ctx->tabs_h) folds.push_back(T.fold);
+  std::string src = pair_source(c, ctx->pot_srcs, ctx->pot_evens, folds);
   if (ctx->mod && src == ctx->mod_key) return DP_OK;
   std::vector<char> code;
   std::string log;
@@ -1296,6 +1353,20 @@ bool collect_block_times(dp_ctx* ctx, int done) {
 DP_API int dp_version(void) { return 100; }
 
 
+// Folded force terms for the compile checks: wherever the mixed kernels with
+// FP64 pair terms can use them (the tables decide at run time).
+std::vector<int> check_folds(const std::vector<std::string>& sv, int precision, int pair_precision) {
+  std::vector<int> f(sv.size(), 0);
+  int n, e[4];
+  double cf[4];
+  for (size_t k = 0; k < sv.size(); ++k)
+    f[k] = (precision == DP_MIXED && !pair_precision && parse_gsk(sv[k], n, e, cf)) ? 1 : 0;
+  return f;
+}
+int check_fold(const char* src, int precision, int pair_precision) {
+  return check_folds(std::vector<std::string>{src}, precision, pair_precision)[0];
+}
+
 // Default plan used by the device-free compile checks.
 PairConfig default_config(int dim, int precision, int ntypes, int even, int pair_precision) {
   PairConfig c{dim, precision, ntypes, even, rs_for_dim(dim), 8, 16, 16,
@@ -1307,7 +1378,8 @@ DP_API int dp_compile_check(int dim, int precision, int ntypes, int even, const 
                             char* log, int64_t log_len, int64_t* code_size) {
   if (dim < 1 || dim > DP_DMAX || !device_src) return DP_ERR_ARG;
   if (ntypes < 1 || ntypes > DP_MAXTYPES) return DP_ERR_ARG;
-  std::string src = pair_source(default_config(dim, precision, ntypes, even, pair_precision), device_src);
+  std::string src = pair_source(default_config(dim, precision, ntypes, even, pair_precision), device_src,
+                                check_fold(device_src, precision, pair_precision));
   std::vector<char> code;
   std::string lg;
   int rc = compile_cached(src, code, lg);
@@ -1333,7 +1405,7 @@ DP_API int dp_compile_check_tuned(int dim, int precision, int ntypes, int even, 
   c.nlw = entry_bits;
   c.ob = entry_bits;
   c.cmask = collapsed_mask;
-  std::string src = pair_source(c, device_src);
+  std::string src = pair_source(c, device_src, check_fold(device_src, precision, pair_precision));
   std::vector<char> code;
   std::string lg;
   int rc = compile_cached(src, code, lg);
@@ -1357,7 +1429,8 @@ DP_API int dp_compile_check_potentials(int dim, int precision, int ntypes, int n
     sv.emplace_back(device_srcs[p]);
     ev.push_back(even[p] ? 1 : 0);
   }
-  std::string src = pair_source(default_config(dim, precision, ntypes, 0, pair_precision), sv, ev);
+  std::string src = pair_source(default_config(dim, precision, ntypes, 0, pair_precision), sv, ev,
+                                check_folds(sv, precision, pair_precision));
   std::vector<char> code;
   std::string lg;
   int rc = compile_cached(src, code, lg);
@@ -1373,7 +1446,8 @@ DP_API int dp_compile_code(int dim, int precision, int ntypes, int even, const c
                            char* out, int64_t out_len, int64_t* code_size) {
   if (dim < 1 || dim > DP_DMAX || !device_src) return DP_ERR_ARG;
   if (ntypes < 1 || ntypes > DP_MAXTYPES) return DP_ERR_ARG;
-  std::string src = pair_source(default_config(dim, precision, ntypes, even, pair_precision), device_src);
+  std::string src = pair_source(default_config(dim, precision, ntypes, even, pair_precision), device_src,
+                                check_fold(device_src, precision, pair_precision));
   std::vector<char> code;
   std::string lg;
   int rc = compile_cached(src, code, lg);
@@ -1640,7 +1714,7 @@ int load_potentials(dp_ctx* ctx, int npot, const char* const* srcs, const int* e
   // compile now (with the default plan) so that errors surface here
   {
     PairConfig c = default_config(ctx->D, ctx->mode, ntypes, 0, pair_precision);
-    std::string src = pair_source(c, sv, ev);
+    std::string src = pair_source(c, sv, ev, check_folds(sv, ctx->mode, pair_precision));
     std::vector<char> code;
     std::string log;
     if (compile_cached(src, code, log) != DP_OK)
@@ -1859,8 +1933,9 @@ DP_API int dp_step(dp_ctx* ctx, int64_t nsteps, int integrator, const double* xi
     return check_ctl(ctx);
   }
   int64_t left = nsteps;
-  // timing brackets every step kernel with events; events recorded inside a
-  // captured graph cannot be timed, so timed runs launch directly
+  // timing brackets the step kernels with events: every step of one step
+  // block in timing_every (the graph variant recording events), or every
+  // single step where graphs cannot time events
   const int timed = ctx->timing ? 1 : 0;
   const bool use_graph = ctx->graph_block > 0 && (!timed || ctx->graph_events_ok) &&
                          nsteps >= (int64_t)ctx->graph_block / 2;
@@ -1872,9 +1947,10 @@ DP_API int dp_step(dp_ctx* ctx, int64_t nsteps, int integrator, const double* xi
     CHK(resolve_timing(ctx));
     while (left > 0) {
       const int par = ctx->cur;
-      if (!ctx->gexec[timed][par] || ctx->gexec_integ[timed][par] != integrator) {
-        const int rc = capture_graph(ctx, integrator, par, timed);
-        if (rc != DP_OK && timed) {
+      const int tb = (timed && ctx->timing_block++ % ctx->timing_every == 0) ? 1 : 0;
+      if (!ctx->gexec[tb][par] || ctx->gexec_integ[tb][par] != integrator) {
+        const int rc = capture_graph(ctx, integrator, par, tb);
+        if (rc != DP_OK && tb) {
           // events inside captured graphs unavailable: time single steps
           ctx->graph_events_ok = false;
           (void)hipGetLastError();
@@ -1883,11 +1959,11 @@ DP_API int dp_step(dp_ctx* ctx, int64_t nsteps, int integrator, const double* xi
         CHK(rc);
       }
       k_block_left<<<1, 64, 0, ctx->st>>>(ctx->ctl_d, left);
-      HIPCHK(hipGraphLaunch(ctx->gexec[timed][par], ctx->st));
+      HIPCHK(hipGraphLaunch(ctx->gexec[tb][par], ctx->st));
       CHK(check_ctl(ctx));  // synchronises; the block's step count and errors
       const int done = ctx->ctl_h->blk_done;
       if (done <= 0 || done > left) return ctx->fail(DP_ERR_STATE, "step block completed %d steps", done);
-      const bool times_ok = !timed || collect_block_times(ctx, done);
+      const bool times_ok = !tb || collect_block_times(ctx, done);
       left -= done;
       if (done & 1) ctx->cur = 1 - ctx->cur;
       if (!times_ok) {
@@ -2262,7 +2338,10 @@ DP_API int dp_set_kernel_timing(dp_ctx* ctx, int enable) {
   if (!ctx) return DP_ERR_ARG;
   (void)hipSetDevice(ctx->dev);
   CHK(resolve_timing(ctx));
+  if (enable < 0) return ctx->fail(DP_ERR_ARG, "kernel timing must be 0 (off) or a block sampling interval >= 1");
   ctx->timing = enable != 0;
+  ctx->timing_every = enable > 0 ? enable : 1;
+  ctx->timing_block = 0;
   ctx->time_ms = 0.0;
   ctx->time_force_ms = 0.0;
   ctx->time_integ_ms = 0.0;

@@ -65,6 +65,7 @@ struct DpPC {
   float ghi, glo;        // mixed screening guards in fixed-point units^2
   float ksf, kef;        // FP32 terms: 2^-2q / sigma^2, eps / sigma * 2^-q
   float epsf, sigf;
+  double gk[4];          // mixed FP64 terms, folded form: coefficients of 1/s^e_k (DpPairTab::gk)
 };
 
 template <class T>
@@ -342,6 +343,10 @@ __device__ __forceinline__ DpPC dp_pc(const DpPairTab& T, int tt) {
   c.kef = T.eps_over_sigma_qf[tt] * (float)DP_GSK;
   c.epsf = T.eps_f[tt];
   c.sigf = T.sigma_f[tt];
+#if DP_FOLD
+#pragma unroll
+  for (int k = 0; k < DP_GSK_N; ++k) c.gk[k] = T.gk[tt * 4 + k];
+#endif
   return c;
 }
 
@@ -485,6 +490,11 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
         const DpPC& pc = DP_PC(tt[c]);
         // entries off the list (padding: s = 0; beyond the cut) are evaluated
         // too and their term is selected away (no trap on inf / NaN)
+#if DP_FOLD
+        // eps f(x)/r per fixed-point unit straight from the squared
+        // fixed-point distance: the scales are in the coefficients
+        const double fs = in[c] ? dp_pot_gsk<double>(sd[c], pc.gk) : 0.0;
+#else
         const double sx = sd[c] * pc.ks;
 #if DP_EVEN
         const double gn = DP_GSN(double, sx);  // f(x)/x / DP_GS_SCALE
@@ -493,9 +503,14 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
         const double gn = dp_pot_f<double>(xr) * dp_rcp(xr);
 #endif
         const double fs = in[c] ? pc.ke * gn : 0.0;  // eps f(x)/r per fixed-point unit
+#endif
 #pragma unroll
         for (int d = 0; d < DP_D; ++d) f[d] = fma(fs, dq[c][d], f[d]);
         if (KIND == 1 && in[c]) {
+#if DP_FOLD
+          const double sx = sd[c] * pc.ks;
+          const double gn = DP_GSN(double, sx);
+#endif
           const double g = gn * DP_GSK;
 #if DP_EVEN
           e += pc.eps * (dp_pot_us<double>(sx) - T.u_rc);

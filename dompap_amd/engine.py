@@ -261,8 +261,10 @@ class Engine:
     def sync(self) -> None:
         self._chk(self.lib.dp_sync(self.ctx))
 
-    def set_kernel_timing(self, enable: bool) -> None:
-        self._chk(self.lib.dp_set_kernel_timing(self.ctx, int(bool(enable))))
+    def set_kernel_timing(self, enable: bool, every: int = 1) -> None:
+        """HIP-event timing of the step kernels; with every = k > 1, of every
+        step of one step block in k (the first one included)."""
+        self._chk(self.lib.dp_set_kernel_timing(self.ctx, int(every) if enable else 0))
 
     def kernel_time(self):
         ms = ctypes.c_double()

@@ -200,10 +200,13 @@ int dp_set_block_rebuild(dp_ctx* ctx, int mode);
  * Simulation.py:726-737, has no graphs). */
 int dp_prepare_step_graphs(dp_ctx* ctx, int integrator, int timed);
 
-/* Kernel timing: when enabled, each fused step-kernel launch is bracketed by
- * HIP events on the context stream (steps are launched directly, not from a
- * captured graph, while timing is on); dp_get_kernel_time returns the summed
- * milliseconds and the number of launches timed since timing was enabled. */
+/* Kernel timing: enable = 0 turns it off; enable = k >= 1 brackets the step
+ * kernels with HIP events on the context stream, recorded by the captured
+ * step-block graph in the first step block and every k-th one after it (k = 1:
+ * every step; a larger k samples whole blocks and keeps the events' own cost
+ * off most steps), or around every step where graphs cannot time events.
+ * dp_get_kernel_time returns the summed milliseconds and the number of
+ * launches timed since timing was enabled. */
 int dp_set_kernel_timing(dp_ctx* ctx, int enable);
 int dp_get_kernel_time(dp_ctx* ctx, double* total_ms, int64_t* launches);
 
