@@ -467,10 +467,17 @@ int refresh_reps(dp_ctx* ctx) {
   }
   // measured on MI355X (3D LJ, ~95 entries per row): 2 lanes per particle
   // and the split step are fastest; the fused kernel idles lanes while one
-  // lane per dimension integrates
+  // lane per dimension integrates.  In mixed mode, one system large enough
+  // for two passes of 256 particles per tile and 8 such tiles per compute
+  // unit takes one lane per particle (3-D LJ at N = 2^20: force kernel
+  // 169.7 -> 164.6 us; 256 replicas of ~5,000 particles -18 % and N = 4,096
+  // -20 % with one lane; the FP64 kernel 506 -> 555 us)
   int tpp = 1;
   if (kbar_max >= 160) tpp = 4;
   else if (kbar_max >= 24) tpp = 2;
+  if (tpp == 2 && ctx->mode == DP_MIXED && ctx->R == 1 &&
+      (double)ctx->N / (2.0 * DP_TILE_THREADS) >= 8.0 * ctx->ncu)
+    tpp = 1;
   if (ctx->tune_tpp > 0) tpp = ctx->tune_tpp;
   ctx->fused = ctx->tune_fused >= 0 ? ctx->tune_fused : 0;
   // home particles per tile: two passes of the workgroup's lane groups
