@@ -998,7 +998,8 @@ __device__ __forceinline__ int dp_emit_group(const DpEmitCell<D, E>& C, int k0) 
         hm |= __builtin_amdgcn_ballot_w64(x);
       }
       if (__builtin_amdgcn_inverse_ballot_w64(hm)) {
-        const u32 e = min(__builtin_amdgcn_mbcnt_hi((u32)(hm >> 32), __builtin_amdgcn_mbcnt_lo((u32)hm, 0u)) + cnt[g],
+        // the running length enters as mbcnt's addend
+        const u32 e = min(__builtin_amdgcn_mbcnt_hi((u32)(hm >> 32), __builtin_amdgcn_mbcnt_lo((u32)hm, cnt[g])),
                           last);
         *(E*)(rows + (u64)((e << ESH) + (u32)g * rstepb)) = (E)idx;
       }
