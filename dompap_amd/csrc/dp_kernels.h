@@ -716,6 +716,19 @@ __device__ __forceinline__ u64 dp_emit_sq(const i32* dd) {
 // when the FP32 screen puts it below the band; inside the screen's slack the
 // exact fixed-point distance decides, and inside the fixed-point band the
 // reference's FP64 test on xref.
+// Rows are padded with zero entries (staged index 0, a valid record of any
+// non-empty tile) from their length c up to the next multiple of the pair
+// kernels' chunk (four 16-bit or two 32-bit entries, one 8-byte word): the
+// pair kernels then read every entry of a row's last chunk without a select
+// (entries past the length are excluded from the sums by their position).
+// Rows are K entries at most; the padding stays within the stride (K
+// rounded up to 8).
+template <typename E>
+__device__ __forceinline__ u32 dp_row_pad(u32 c) {
+  constexpr u32 EPW = 8 / sizeof(E);
+  return (EPW - c % EPW) % EPW;
+}
+
 template <int D> struct DpEmitG { static constexpr int value = D >= 4 ? 4 : 8; };
 template <int D, typename E, bool LDS, bool CM>
 __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const DpRep& P, const double* xref,
@@ -892,6 +905,9 @@ __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const 
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         if (g < nj) {
+          // zero entries up to the pair kernels' chunk of the row's end
+          const u32 c0 = min(cnt[g], (u32)K);
+          if ((u32)lane < dp_row_pad<E>(c0)) nl[(slot0 + (selfv[g] - self0)) * Ks + c0 + lane] = (E)0;
           if (lane == 0) {
             nlcnt[slot0 + (selfv[g] - self0)] = (i32)cnt[g];
             if (cnt[g] > (u32)K) ctl->overflow = 1;
@@ -1009,6 +1025,9 @@ __device__ __forceinline__ int dp_emit_group(const DpEmitCell<D, E>& C, int k0) 
   int mx = 0;
 #pragma unroll
   for (int g = 0; g < NJ; ++g) {
+    // zero entries up to the pair kernels' chunk of the row's end
+    const u32 c0 = min(cnt[g], (u32)C.K);
+    if ((u32)lane < dp_row_pad<E>(c0)) *(E*)(rows + (u64)(((c0 + (u32)lane) << ESH) + (u32)g * rstepb)) = (E)0;
     if (lane == 0) {
       C.nlcnt[C.slot0 + k0 + NW * g] = (i32)cnt[g];
       if (cnt[g] > (u32)C.K) C.ctl->overflow = 1;
@@ -1149,7 +1168,8 @@ __device__ __forceinline__ void dp_emit_lanes(const DpTileSmem<D>& S, const DpRe
         }
       }
     }
-    // the last, partial group (the entries past cnt are not read)
+    // the last, partial group, its entries past cnt zero (the pad of
+    // dp_row_pad)
     const u32 g0 = cnt - cnt % EPW;
     if (cnt % EPW != 0 && g0 < (u32)K) *(u64*)(row + g0) = pend;
     nlcnt[slot] = (i32)cnt;

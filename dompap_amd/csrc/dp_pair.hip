@@ -117,7 +117,8 @@ __device__ __forceinline__ DpLoad dp_ld_entries(const dp_ent_t* p) {
 // Call fn(idx, nvalid) for the chunks of a row of length cnt that belong to
 // lane `sub` of its group of tpp lanes: loads of DP_CPL chunks at entries
 // [W * (sub + j * tpp), + W), W = DP_EPL * DP_CPL, for j = 0, 1, ...; idx[c]
-// for c >= nvalid is unspecified.
+// for c >= nvalid is 0: the emit pads every row with zero entries up to a
+// multiple of DP_EPL, and a row's last chunk ends within that padding.
 template <class Fn>
 __device__ __forceinline__ void dp_for_chunks(const dp_ent_t* row, int cnt, int sub, int tpp, Fn& fn) {
   constexpr int W = DP_EPL * DP_CPL;
@@ -430,7 +431,9 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
       DpRec rj[DP_EPL];
 #pragma unroll
       for (int c = 0; c < DP_EPL; ++c) {
-        const int ic = (c < nvalid) ? idx[c] : self;  // padding entries read the particle itself
+        // entries past the row's end are the emit's zero padding (staged
+        // index 0), read like the others and excluded from the sums below
+        const int ic = idx[c];
         if (LDS) {
           rj[c] = dp_ld_lrec(srec + ic * DP_LS);
         } else {
@@ -561,7 +564,7 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
       int tj[DP_EPL];
 #pragma unroll
       for (int c = 0; c < DP_EPL; ++c) {
-        const int ic = (c < nvalid) ? idx[c] : self;
+        const int ic = idx[c];  // past the row's end: the emit's zero padding
         tj[c] = 0;
         if (LDS) {
 #pragma unroll
