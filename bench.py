@@ -206,6 +206,18 @@ def sync_all(eng):
         pass
 
 
+def tune_cells(eng, args, log):
+    """With --cell-scale 0, the cell width chosen by dompap_amd's autotuner
+    (timed steps at each width before the warmup); else None."""
+    if args.cell_scale:
+        return None
+    from dompap_amd.tools.autotune import tune_cell_scale
+    scale, times = tune_cell_scale(eng, steps=args.tune_steps)
+    per = {f"{s:g}": round(t / args.tune_steps * 1e3, 5) for s, t in times.items()}
+    log(f"cell width {scale:g} list cutoffs (ms per step by width: {per})")
+    return {"chosen": scale, "ms_per_step_by_scale": per, "steps_each": args.tune_steps}
+
+
 def timed_run(eng, steps, timing=True, breakdown=False):
     """Wall time of `steps` G-JF steps (device work bracketed by stream
     synchronisations), the HIP-event time of the step kernels, launches and
@@ -346,6 +358,7 @@ def run_replicas(args, log):
     if args.tpp or args.tile or args.fused >= 0:
         tuning = (args.tpp, args.tile, args.fused)
     eng = make_batch_engine(wl, args.precision, tuning=tuning)
+    cell_tuning = tune_cells(eng, args, log)
     eng.step(args.warmup, "gjf")
     eng.prepare_step_graphs("gjf", timed=True)
     eng.prepare_step_graphs("gjf", timed=False)
@@ -390,6 +403,7 @@ def run_replicas(args, log):
                      "mean_list_length": kbar, "avg_launch_us": force_s * 1e6,
                      "integrate_kernel": {"kernel": "dp_integrate_gjf", "avg_launch_us": integ_s * 1e6}},
         "plan": plan,
+        "cell_tuning": cell_tuning,
         "timing": (f"HIP events recorded inside the step-block graphs of one block in {TIMING_EVERY} "
                    "(100-step blocks, per-replica rebuilds)"),
         "graph_replay_value": n * args.steps / dtg,
@@ -400,6 +414,7 @@ def run_replicas(args, log):
     if not args.no_fp64 and args.precision == "mixed":
         log("FP64 comparison run")
         e64 = make_batch_engine(wl, "fp64")
+        cell_tuning64 = tune_cells(e64, args, log)
         e64.step(args.warmup, "gjf")
         e64.prepare_step_graphs("gjf", timed=False)
         k64 = max(args.steps // 2, 200)
@@ -407,6 +422,8 @@ def run_replicas(args, log):
         v64 = n * k64 / dt64
         out["fp64_value"] = v64
         out["mixed_over_fp64"] = value / v64
+        if cell_tuning64 is not None:
+            out["fp64_cell_tuning"] = cell_tuning64
         log(f"fp64: {v64:.4e} particle-steps/s (ratio {value / v64:.2f})")
         e64.set_field("positions", x_now)
         r64 = e64.compute(forces=True, energy=True, virial=True)
@@ -471,7 +488,10 @@ def main():
     ap.add_argument("--tpp", type=int, default=0, help="lanes per particle (0: planner's choice)")
     ap.add_argument("--tile", type=int, default=0, help="home particles per tile (0: planner's choice)")
     ap.add_argument("--cell-scale", type=float, default=1.0,
-                    help="cells at least this many list cutoffs wide (1.0: the narrowest)")
+                    help="cells at least this many list cutoffs wide (1.0: the planner's narrowest, the default); "
+                         "0: dompap_amd's autotuner chooses, timing each width on the workload before the warmup "
+                         "(in both precision modes)")
+    ap.add_argument("--tune-steps", type=int, default=200, help="timed steps per cell width while autotuning")
     ap.add_argument("--fused", type=int, default=-1, choices=[-1, 0, 1],
                     help="1: fused force + integrate kernel, 0: force kernel then integrate kernel, -1: planner")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
@@ -502,7 +522,8 @@ def main():
     tuning = None
     if args.tpp or args.tile or args.fused >= 0:
         tuning = (args.tpp, args.tile, args.fused)
-    eng = make_engine(wl, args.precision, args.pair_precision, tuning, args.cell_scale)
+    eng = make_engine(wl, args.precision, args.pair_precision, tuning, args.cell_scale or 1.0)
+    cell_tuning = tune_cells(eng, args, log)
     eng.step(args.warmup, "gjf")
     # every step-block graph the timed region replays (events recorded or
     # not, either buffer parity) is captured and instantiated here, outside
@@ -587,6 +608,7 @@ def main():
                                           "bytes_per_particle": b_integ,
                                           "achieved_GBs": n * b_integ / integ_s / 1e9 if integ_s > 0 else None}},
         "plan": plan,
+        "cell_tuning": cell_tuning,
         "timing": (f"HIP events recorded inside the step-block graphs of one block in {TIMING_EVERY}"
                    if eng.plan()["graph_events"] else "HIP events around single-step launches"),
         "graph_replay_value": n * args.steps / dtg,
@@ -596,14 +618,18 @@ def main():
     x_now = eng.get_field("positions")
     if not args.no_fp64 and args.precision == "mixed":
         log("FP64 comparison run")
-        eng64 = make_engine(wl, "fp64")
+        eng64 = make_engine(wl, "fp64", cell_scale=args.cell_scale or 1.0)
+        cell_tuning64 = tune_cells(eng64, args, log)
         eng64.step(args.warmup, "gjf")
         eng64.prepare_step_graphs("gjf", timed=True)
+        eng64.prepare_step_graphs("gjf", timed=False)
         k64 = max(args.steps // 4, 50)
         dt64, ms64, l64, rb64 = timed_run(eng64, k64, timing=True)
         v64 = n * k64 / dt64
         out["fp64_value"] = v64
         out["mixed_over_fp64"] = value / v64
+        if cell_tuning64 is not None:
+            out["fp64_cell_tuning"] = cell_tuning64
         out["fp64_step_kernel_us"] = ms64 / max(l64, 1) * 1e3
         log(f"fp64: {v64:.4e} particle-steps/s (ratio {value / v64:.2f})")
         # errors of the mixed kernels on the mixed run's configuration

@@ -1037,19 +1037,71 @@ __device__ __forceinline__ int dp_emit_group(const DpEmitCell<D, E>& C, int k0) 
   return mx;
 }
 
+// A home cell's box in its tile's fixed-point frame, in the high 16 bits of
+// the coordinates (the 16-bit screen's units): centre, half width widened
+// so that the gap computed below is a lower bound of the true gap / 2^16
+// (each high-half difference is within 1 of the true one / 2^16), and the
+// squared distance in those units beyond which no pair is listed.
+typedef unsigned short dp_u16x2 __attribute__((ext_vector_type(2)));
+struct DpEmitBox {
+  u32 cxy, czw;       // centre (the cell's shift in the tile frame), high halves packed (x, y) and (z, w)
+  dp_u16x2 hxy, hzw;  // half widths + slack, same packing
+  u32 thr;            // P.nlq_hi / 2^32
+};
+template <int D>
+__device__ __forceinline__ DpEmitBox dp_emit_box(const DpTileSmem<D>& S, const DpRep& P, int uh) {
+  u32 c[4] = {0u, 0u, 0u, 0u};
+  unsigned short h[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    c[d] = __builtin_amdgcn_readfirstlane(dp_halo_shift<D>(S, P, uh, d));
+    h[d] = (unsigned short)min((((u32)(P.Wq[d] / 2) + 4u + 0xffffu) >> 16) + 2u, 0xffffu);
+  }
+  DpEmitBox b;
+  b.cxy = dp_pack_hi16((i32)c[0], (i32)c[1]);
+  b.czw = dp_pack_hi16((i32)c[2], (i32)c[3]);
+  b.hxy = dp_u16x2{h[0], h[1]};
+  b.hzw = dp_u16x2{h[2], h[3]};
+  const u64 t = P.nlq_hi >> 32;
+  b.thr = t > 0xffffffffull ? 0xffffffffu : (u32)t;
+  return b;
+}
+// Whether a staged record can be within the list cutoff of a point of the
+// box: per dimension the gap max(|high-half difference| - half width, 0)
+// (packed 16-bit operations), their squares summed with saturation.  A gap is
+// at most the true gap / 2^16 and a saturated sum is smaller than the true
+// one, so a record is dropped only when every pair with the box's particles
+// lies beyond P.nlq_hi (the fixed-point bound above which no pair is listed).
+template <int D>
+__device__ __forceinline__ bool dp_emit_in_box(const int4 q, const DpEmitBox& b) {
+  const dp_s16x2 dxy = __builtin_bit_cast(dp_s16x2, dp_pack_hi16(q.x, q.y)) - __builtin_bit_cast(dp_s16x2, b.cxy);
+  const dp_u16x2 gxy = __builtin_elementwise_sub_sat(__builtin_bit_cast(dp_u16x2, __builtin_elementwise_max(dxy, -dxy)),
+                                                     b.hxy);
+  u32 s = __builtin_amdgcn_udot2(gxy, gxy, 0u, true);
+  if (D >= 3) {
+    const dp_s16x2 dzw = __builtin_bit_cast(dp_s16x2, dp_pack_hi16(q.z, q.w)) - __builtin_bit_cast(dp_s16x2, b.czw);
+    const dp_u16x2 gzw = __builtin_elementwise_sub_sat(
+        __builtin_bit_cast(dp_u16x2, __builtin_elementwise_max(dzw, -dzw)), b.hzw);
+    s = __builtin_amdgcn_udot2(gzw, gzw, s, true);
+  }
+  return s <= b.thr;
+}
+
 // Emit of one tile's rows (up to 8 particles per candidate pass) for tiles staged in LDS with no single-cell
 // dimension.  Same rows, in the same order, as dp_emit_particles.  Per home
 // cell, the workgroup first writes the candidate list (staged indices, in
-// stencil-row order) to an LDS table `ctab`, so that a lane finds its
-// candidate with one LDS read.  The group's particle index, row pointer and
+// stencil-row order, without the records that the home cell's box puts
+// beyond the list cutoff of all its particles) to an LDS table `ctab`, so
+// that a lane finds its candidate with one LDS read.  The group's particle index, row pointer and
 // running row length are wavefront-uniform (scalar registers); the screen's
 // results stay lane masks: a hit is stored under its own compare, at
 // mbcnt(mask) after the running length.  The self entry sits at a known
 // candidate position (the home cell is the centre of its stencil row).
+#define DP_EMIT_NROW 27  // stencil rows of D <= 4
 template <int D, typename E>
 __device__ __forceinline__ void dp_emit_particles_tab(const DpTileSmem<D>& S, const DpRep& P, const double* xref,
-                                                      const int4* srq, u16* ctab, int K, E* nl, i32* nlcnt,
-                                                      DpCtl* ctl, int& st_count) {
+                                                      const int4* srq, u16* ctab, i32* ekeep, int K, E* nl,
+                                                      i32* nlcnt, DpCtl* ctl, int& st_count) {
   constexpr int N3 = DpPow3<D>::value;
   constexpr int NROW = N3 / 3;
   constexpr int G = 8;
@@ -1079,22 +1131,54 @@ __device__ __forceinline__ void dp_emit_particles_tab(const DpTileSmem<D>& S, co
       r_lo = S.hbase[u0];
       r_len = S.hbase[uh + S.st_off[3 * lane + 2] + 1] - r_lo;
     }
-    int incl = r_len;
+    // candidates: the stencil rows' records that can lie within the list
+    // cutoff of some particle of the home cell (the box test), in
+    // stencil-row order; with cells 1.1 list cutoffs wide, ~62 % of the
+    // stencil in 3-D.  Pass 1 counts each row's (wavefront w: rows w,
+    // w + NW, ..; in the home cell's row also those before the home cell,
+    // whose own particles are all kept)
+    const DpEmitBox box = dp_emit_box<D>(S, P, uh);
+    const int hb = __builtin_amdgcn_readfirstlane(S.hbase[uh]);
+    for (int r = wave; r < NROW; r += NW) {
+      const int rl = __builtin_amdgcn_readlane(r_lo, r);
+      const int rn = __builtin_amdgcn_readlane(r_len, r);
+      int kept = 0, before = 0;
+      for (int i0 = 0; i0 < rn; i0 += 64) {
+        const int i = i0 + lane;
+        const bool k = i < rn && dp_emit_in_box<D>(srq[rl + i], box);
+        kept += __popcll(__builtin_amdgcn_ballot_w64(k));
+        if (r == RC) before += __popcll(__builtin_amdgcn_ballot_w64(k && rl + i < hb));
+      }
+      if (lane == 0) {
+        ekeep[r] = kept;
+        if (r == RC) ekeep[DP_EMIT_NROW] = before;
+      }
+    }
+    __syncthreads();
+    int incl = (lane < NROW) ? ekeep[lane] : 0;
+    const int r_keep = incl;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       const int vv = __shfl_up(incl, off);
       if (lane >= off) incl += vv;
     }
-    const int r_pref = incl - r_len;
+    const int r_pref = incl - r_keep;
     const int ncand = __builtin_amdgcn_readlane(incl, NROW - 1);
     // candidate position of the home cell's first particle
-    const int cs0 = __builtin_amdgcn_readlane(r_pref, RC) + (self0 - __builtin_amdgcn_readlane(r_lo, RC));
-    // the table: row r's staged indices at r_pref[r] ..; wavefront w writes rows w, w + NW, ..
+    const int cs0 = __builtin_amdgcn_readlane(r_pref, RC) + ekeep[DP_EMIT_NROW] + (self0 - hb);
+    // pass 2, the table: row r's kept staged indices at r_pref[r] ..
     for (int r = wave; r < NROW; r += NW) {
       const int rl = __builtin_amdgcn_readlane(r_lo, r);
       const int rn = __builtin_amdgcn_readlane(r_len, r);
-      const int rp = __builtin_amdgcn_readlane(r_pref, r);
-      for (int i = lane; i < rn; i += 64) ctab[rp + i] = (u16)(rl + i);
+      int pos = __builtin_amdgcn_readlane(r_pref, r);
+      for (int i0 = 0; i0 < rn; i0 += 64) {
+        const int i = i0 + lane;
+        const bool k = i < rn && dp_emit_in_box<D>(srq[rl + i], box);
+        const u64 km = __builtin_amdgcn_ballot_w64(k);
+        if (k) ctab[__builtin_amdgcn_mbcnt_hi((u32)(km >> 32), __builtin_amdgcn_mbcnt_lo((u32)km, (u32)pos))] =
+            (u16)(rl + i);
+        pos += __popcll(km);
+      }
     }
     __syncthreads();
     // this wavefront's particles: k = wave + NW * j, j = 0 .. npw - 1, in
@@ -1196,6 +1280,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* 
   __shared__ DpTileSmem<D> S;
   extern __shared__ __align__(16) int4 srq[];  // cap staged records, then cap u16 candidate indices
   u16* ctab = (u16*)(srq + cap);
+  __shared__ i32 ekeep[DP_EMIT_NROW + 1];  // candidates kept per stencil row; those before the home cell
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -1228,7 +1313,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* 
     if (fits) {
       if (cm) dp_emit_particles<D, E, true, true>(S, P, xref, cellpack, qinv, srq, K, nl, nlcnt, ctl, st_count);
       else if (lanes) dp_emit_lanes<D, E>(S, P, xref, srq, K, nl, nlcnt, ctl, st_count);
-      else dp_emit_particles_tab<D, E>(S, P, xref, srq, ctab, K, nl, nlcnt, ctl, st_count);
+      else dp_emit_particles_tab<D, E>(S, P, xref, srq, ctab, ekeep, K, nl, nlcnt, ctl, st_count);
     }
     __syncthreads();
   }

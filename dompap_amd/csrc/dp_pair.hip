@@ -998,5 +998,5 @@ __device__ __forceinline__ void dp_integrate_body(const DpPairArgs& a) {
   }
 }
 
-extern "C" __global__ void __launch_bounds__(256) dp_integrate_gjf(DpPairArgs a) { dp_integrate_body<1>(a); }
-extern "C" __global__ void __launch_bounds__(256) dp_integrate_lf(DpPairArgs a) { dp_integrate_body<2>(a); }
+extern "C" __global__ void __launch_bounds__(256, 5) dp_integrate_gjf(DpPairArgs a) { dp_integrate_body<1>(a); }
+extern "C" __global__ void __launch_bounds__(256, 5) dp_integrate_lf(DpPairArgs a) { dp_integrate_body<2>(a); }

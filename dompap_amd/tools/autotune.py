@@ -20,7 +20,26 @@ from __future__ import annotations
 
 from time import perf_counter
 
-CELL_SCALES = (1.0, 1.15, 1.3)
+CELL_SCALES = (1.0, 1.05, 1.1, 1.15, 1.2, 1.3)
+
+
+def tune_cell_scale(engine, steps=200, scales=CELL_SCALES, integrator="gjf"):
+    """Cell width of an engine's system (cells at least ``scale`` list
+    cutoffs wide; larger cells make fewer, larger tiles and longer candidate
+    lists): ``steps`` timed steps at each width, the fastest kept.  Each
+    width's new cells, list, plan and step graphs are made outside its timed
+    run.  Returns (scale, {scale: seconds})."""
+    times = {}
+    for scale in scales:
+        engine.set_cell_scale(scale)
+        engine.step(max(steps // 10, 8), integrator)
+        engine.prepare_step_graphs(integrator, timed=False)
+        start = perf_counter()
+        engine.step(steps, integrator)  # returns after the device finished
+        times[scale] = perf_counter() - start
+    best = min(times, key=times.get)
+    engine.set_cell_scale(best)
+    return best, times
 
 
 def _seconds(sim, steps):
