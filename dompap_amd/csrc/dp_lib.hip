@@ -359,6 +359,12 @@ struct dp_ctx {
   bool capturing = false;               // a step block is being captured (events become graph nodes)
   bool graph_dirty = true;
 
+  // largest staged halo (records) measured by an emit so far: raises the
+  // LDS record areas in inhomogeneous systems; the plan's mean halo
+  // occupancy and staged record size
+  int halo_seen = 0;
+  double halo_occ = 0.0;
+  size_t rec_bytes = 0;
   // kernel timing
   bool timing = false;
   int timing_every = 1;   // step blocks: events in one block of every timing_every
@@ -445,6 +451,38 @@ double ball_volume(int D) {
 //    halos are small enough, else 32-bit.
 // A new plan over the same cells re-emits the existing list (from xref)
 // instead of rebuilding it, so rebuild times do not depend on the plan.
+// LDS record areas of the pair kernels and the emit: the mean halo plus
+// three standard deviations of a Poisson count (at most 1.5x the mean) and
+// 64, at most 48 KB; the rare fuller tiles take the global-memory paths.
+// (4-D tiles: 2,027 records instead of 2,817, which brings the force kernel
+// to 4 workgroups per CU.)  A larger halo measured by an emit (inhomogeneous
+// systems: the dense tiles) raises both areas as far as four workgroups per
+// compute unit allow.  Launch sizes only: the list and the plan stay valid.
+void size_lds(dp_ctx* ctx) {
+  const double h = ctx->halo_occ;
+  const i64 stat = (i64)std::ceil(std::min(1.5 * h, h + 3.0 * std::sqrt(h))) + 64;
+  const i64 want = (i64)ctx->halo_seen + 32;
+  // pair kernels: 40 KB of LDS per workgroup at four per CU, ~2 KB of it the
+  // tile tables
+  i64 cap = stat;
+  if (want > cap) cap = std::max<i64>(cap, std::min<i64>(want, (i64)((38 * 1024) / ctx->rec_bytes)));
+  cap = std::min<i64>(cap, (i64)(48 * 1024 / ctx->rec_bytes));
+  cap = std::max<i64>(cap, 64);
+  if ((int)cap != ctx->cap) release_graph(ctx);  // captured launches carry the LDS size
+  ctx->cap = (int)cap;
+  ctx->lds_pair = (size_t)cap * ctx->rec_bytes;
+  // emit: 18 bytes per record (an int4 record and a u16 candidate-table
+  // entry), ~7.3 KB of tile tables; with 8-cell tiles four emit workgroups
+  // fit a compute unit instead of three
+  i64 ce = stat;
+  if (want > ce) ce = std::max<i64>(ce, std::min<i64>(want, (i64)((40 * 1024 - 7680) / 18)));
+  ce = std::min<i64>(ce, (i64)(48 * 1024 / 16));
+  ce = std::max<i64>(ce, 64);
+  if ((int)ce != ctx->cap_emit) release_graph(ctx);
+  ctx->cap_emit = (int)ce;
+  ctx->lds_emit = (size_t)ce * 16 + ((size_t)ce * 2 + 15) / 16 * 16;
+}
+
 int launch_records(dp_ctx* ctx);
 int launch_reemit(dp_ctx* ctx);
 int refresh_reps(dp_ctx* ctx) {
@@ -702,13 +740,6 @@ int refresh_reps(dp_ctx* ctx) {
   int nlw = ctx->nlw_min;
   if (halo_occ_max * 4.0 + 64.0 >= 65536.0) nlw = 32;
   const int ob = nlw;
-  // LDS records: the mean halo plus three standard deviations of a Poisson
-  // count (at most 1.5x the mean) and 64, at most 48 KB; the rare fuller
-  // tiles take the global-memory path.  (4-D tiles: 2,027 records instead
-  // of 2,817, which brings the force kernel to 4 workgroups per CU.)
-  i64 cap = (i64)std::ceil(std::min(1.5 * halo_occ_max, halo_occ_max + 3.0 * std::sqrt(halo_occ_max))) + 64;
-  cap = std::min<i64>(cap, (i64)(48 * 1024 / rec_bytes));
-  cap = std::max<i64>(cap, 64);
   // geometry of the built list: cells (a change needs a rebuild) and tiles /
   // entry width / fixed-point scale (a change needs a re-emit from xref)
   std::vector<i64> cells_sig, tiles_sig;
@@ -728,16 +759,10 @@ int refresh_reps(dp_ctx* ctx) {
   ctx->nlw = nlw;
   ctx->ob = ob;
   ctx->tpp = tpp;
-  ctx->cap = (int)cap;
-  ctx->lds_pair = (size_t)cap * rec_bytes;
-  {
-    // the emit's record area by the same rule (mean halo plus three Poisson
-    // standard deviations): with 8-cell tiles four emit workgroups fit a
-    // compute unit instead of three
-    i64 ce = (i64)std::ceil(std::min(1.5 * halo_occ_max, halo_occ_max + 3.0 * std::sqrt(halo_occ_max))) + 64;
-    ce = std::min<i64>(ce, (i64)(48 * 1024 / 16));
-    ctx->cap_emit = (int)std::max<i64>(ce, 64);
-  }
+  ctx->halo_occ = halo_occ_max;
+  ctx->rec_bytes = rec_bytes;
+  if (cells_changed) ctx->halo_seen = 0;  // halos measured on other cells
+  size_lds(ctx);
   {
     // cells of a few particles: one emit lane per particle (the
     // candidate-table pass takes a wavefront per home cell and group)
@@ -745,8 +770,6 @@ int refresh_reps(dp_ctx* ctx) {
     if (lanes != ctx->emit_lanes) release_graph(ctx);  // captured emit launches carry the mode
     ctx->emit_lanes = lanes;
   }
-  // one int4 record and one u16 candidate-table entry per staged particle
-  ctx->lds_emit = (size_t)ctx->cap_emit * 16 + ((size_t)ctx->cap_emit * 2 + 15) / 16 * 16;
   ctx->ntiles = t0;
   {
     // one workgroup per tile (measured: 209 -> 202 us for the 3-D LJ force
@@ -1156,6 +1179,11 @@ int check_ctl(dp_ctx* ctx) {
   HIPCHK(hipMemcpyAsync(ctx->ctl_h, ctx->ctl_d, sizeof(DpCtl), hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   reduce_ctl(ctx->ctl_h);
+  if (ctx->ctl_h->max_halo > ctx->halo_seen) {
+    // launches after this read-back size their LDS record areas for it
+    ctx->halo_seen = ctx->ctl_h->max_halo;
+    if (ctx->rec_bytes) size_lds(ctx);
+  }
   ctx->last_cell_overflow = false;
   if (ctx->ctl_h->overflow || ctx->ctl_h->nonfinite || ctx->ctl_h->cell_overflow) {
     const bool ov = ctx->ctl_h->overflow != 0;

@@ -440,9 +440,10 @@ def test_fused_and_split_step_kernels_agree(precision):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_inhomogeneous_tiles_beyond_lds_capacity(precision):
-    """Half of the box empty: the planner sizes the LDS record area from the
+    """Half of the box empty: the planner sizes the LDS record areas from the
     mean occupancy, so the dense tiles take the global-memory paths of the
-    emit and the pair kernels; lists and forces must not change."""
+    emit and the pair kernels (the densest still do after the measured halo
+    raised the areas); lists and forces must not change."""
     from dompap_amd import Simulation
     rng = np.random.default_rng(31)
     g = np.arange(12)
@@ -471,6 +472,14 @@ def test_inhomogeneous_tiles_beyond_lds_capacity(precision):
     # a tile inside the occupied half holds about twice the mean occupancy
     dense_halo = 2.0 * x.shape[0] / plan["cells"] * plan["halo_cells_per_tile"]
     assert dense_halo > plan["lds_records"]
+    # the largest halo that the emit measured raises the LDS record areas
+    # of the later launches above the mean-occupancy rule (as far as four
+    # workgroups per compute unit allow); the densest tiles still exceed them
+    h = x.shape[0] / plan["cells"] * plan["halo_cells_per_tile"]
+    rule = math.ceil(min(1.5 * h, h + 3 * math.sqrt(h))) + 64
+    assert plan["lds_records"] > rule and plan["lds_records_emit"] > rule
+    F2 = s.get_forces()
+    assert np.abs(F2 - Fo).max() / _frms(Fo) < tol
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
