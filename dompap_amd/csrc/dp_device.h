@@ -26,6 +26,24 @@ typedef uint16_t u16;
 #define DP_HALO_MAX 512    // halo cells per tile
 #define DP_HOME_MAX 128    // home cells per tile
 #define DP_TILE_THREADS 256
+// bytes allocated after the neighbour list: a lane's row walk loads the
+// next 16-byte group (8 x lanes-per-particle entries on) before it tests
+// whether its row goes on
+#define DP_NL_SLACK 4096
+
+// Neighbour-entry format efmt = esh | sent << 8: an entry is a staged-halo
+// index shifted left by esh (in mixed precision the byte offset of the
+// neighbour's record in the pair kernels' LDS record area); with sent, rows
+// are padded with the entry of a sentinel record staged after the halo.  A
+// tile takes this format when its halo fits the pair kernels' LDS record
+// area at the emit (word 7 of its table); other tiles keep plain indices
+// padded with 0 (efmt 0).  Largest staged halo (records) that 16-bit entries
+// of a format address: entries reach nrec << esh with a sentinel,
+// (nrec - 1) << esh without.
+__host__ __device__ __forceinline__ int dp_efmt_hlim(int efmt) {
+  const int esh = efmt & 31;
+  return (efmt >> 8) ? (65535 >> esh) : (65535 >> esh) + 1;
+}
 
 // One independent system ("replica").  Particles of replica r occupy slots
 // [p0, p0+n) of every per-particle array; its cells occupy [c0, c0+ncell);
@@ -308,6 +326,7 @@ struct DpTileSmem {
   i32 k0, k1;                  // this tile's range of them (a slice; all of them when nslice == 1)
   i32 rep;                     // replica of the tile
   i32 fits;                    // 1 if the halo fits in the LDS record area
+  i32 ebo;                     // 1 if the tile's entries are byte offsets (table word 7)
 };
 
 // neighbor_list.py:183-201 trigger for one particle: squared displacement
@@ -538,7 +557,9 @@ __device__ void dp_tile_setup(DpTileSmem<D>& S, const DpRep* reps, int R, const 
 // build (cell contents are fixed between builds) and loaded by the per-step
 // kernels with one coalesced copy instead of recomputing them:
 //   [0] nrec [1] nhomep [2] rep [3] maxc [4] nhalo [5] nhome
-//   [6] 1 if the halo exceeds the emit's LDS record area [7] unused,
+//   [6] 1 if the halo exceeds the emit's LDS record area [7] 1 if the
+//   tile's neighbour entries are byte offsets of staged records (else plain
+//   staged-halo indices),
 //   [8] [9] the tile's slice [k0, k1) of the home particles [10] [11] unused,
 //   then hstart[nh], hbase[nh + 1], hcoord[nh], home_u[nhm], home_p[nhm + 1].
 #define DP_TT_HEAD 12
@@ -571,8 +592,10 @@ __device__ void dp_tile_store(const DpTileSmem<D>& S, const DpRep& P, i32* tt) {
 }
 
 // Load tile t's tables (every thread calls this; it ends with a barrier).
+// S.fits: the halo fits cap records, and with need_bo the tile's entries are
+// byte offsets (the pair kernels' LDS path reads records by them).
 template <int D>
-__device__ void dp_tile_load(DpTileSmem<D>& S, const i32* tt, int cap) {
+__device__ void dp_tile_load(DpTileSmem<D>& S, const i32* tt, int cap, bool need_bo = false) {
   const int tid = threadIdx.x;
   const int4 h0 = *(const int4*)tt;
   const int4 h1 = *(const int4*)(tt + 4);
@@ -593,7 +616,8 @@ __device__ void dp_tile_load(DpTileSmem<D>& S, const i32* tt, int cap) {
     S.k1 = h2.y;
     S.rep = h0.z;
     S.maxc = h0.w;
-    S.fits = h0.x <= cap ? 1 : 0;
+    S.ebo = h1.w;
+    S.fits = (h0.x <= cap && (!need_bo || h1.w != 0)) ? 1 : 0;
   }
   __syncthreads();
 }

@@ -729,13 +729,25 @@ __device__ __forceinline__ u32 dp_row_pad(u32 c) {
   return (EPW - c % EPW) % EPW;
 }
 
+// Entry format (efmt = esh | sent << 8): an entry is a neighbour's staged-halo
+// index shifted left by esh (mixed precision: the byte offset of its record
+// in the pair kernels' LDS record area); with sent, rows are padded with the
+// entry of the tile's sentinel record (staged index nrec, after the halo),
+// which lies beyond the cut of every home particle, otherwise with 0.
+__device__ __forceinline__ int dp_efmt_esh(int efmt) { return efmt & 31; }
+__device__ __forceinline__ u32 dp_efmt_pad(int efmt, int nrec) {
+  return (efmt >> 8) ? ((u32)nrec << (efmt & 31)) : 0u;
+}
+
 template <int D> struct DpEmitG { static constexpr int value = D >= 4 ? 4 : 8; };
 template <int D, typename E, bool LDS, bool CM>
 __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const DpRep& P, const double* xref,
                                                   const i32* cellpack, double qinv, const int4* srq, int K, E* nl,
-                                                  i32* nlcnt, DpCtl* ctl, int& st_count) {
+                                                  i32* nlcnt, DpCtl* ctl, int efmt, int& st_count) {
   constexpr int N3 = DpPow3<D>::value;
   constexpr int NROW = N3 / 3;  // <= 27 for D <= 4: one row per lane
+  const int esh = dp_efmt_esh(efmt);
+  const u32 pv = dp_efmt_pad(efmt, S.nrec);
   constexpr int G = DpEmitG<D>::value;  // particles per candidate pass
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -897,7 +909,7 @@ __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const 
             const u64 mask = __ballot(h);
             const u32 below = __builtin_amdgcn_mbcnt_hi((u32)(mask >> 32), __builtin_amdgcn_mbcnt_lo((u32)mask, 0u));
             const u32 pos = cnt[g] + below;
-            if (h && pos < (u32)K) nl[(slot0 + (selfv[g] - self0)) * Ks + pos] = (E)idx;
+            if (h && pos < (u32)K) nl[(slot0 + (selfv[g] - self0)) * Ks + pos] = (E)((u32)idx << esh);
             cnt[g] += (u32)__popcll(mask);
           }
         }
@@ -905,9 +917,9 @@ __device__ __forceinline__ void dp_emit_particles(const DpTileSmem<D>& S, const 
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         if (g < nj) {
-          // zero entries up to the pair kernels' chunk of the row's end
+          // padding entries up to the pair kernels' chunk of the row's end
           const u32 c0 = min(cnt[g], (u32)K);
-          if ((u32)lane < dp_row_pad<E>(c0)) nl[(slot0 + (selfv[g] - self0)) * Ks + c0 + lane] = (E)0;
+          if ((u32)lane < dp_row_pad<E>(c0)) nl[(slot0 + (selfv[g] - self0)) * Ks + c0 + lane] = (E)pv;
           if (lane == 0) {
             nlcnt[slot0 + (selfv[g] - self0)] = (i32)cnt[g];
             if (cnt[g] > (u32)K) ctl->overflow = 1;
@@ -959,6 +971,8 @@ template <int D, typename E> struct DpEmitCell {
   int self0;
   i64 slot0;
   int ncand, cs0, lo16, hi16;
+  int esh;  // entry = staged index << esh
+  u32 pv;   // padding entry
 };
 
 // Rows of NJ particles k0, k0 + NW, .. of one home cell against its candidate
@@ -997,6 +1011,7 @@ __device__ __forceinline__ int dp_emit_group(const DpEmitCell<D, E>& C, int k0) 
     const int4 qj = C.srq[idx];
     const u32 cxy = dp_pack_hi16(qj.x, qj.y), czw = dp_pack_hi16(qj.z, qj.w);
     const u64 vmask = __builtin_amdgcn_ballot_w64(valid);
+    const E ev = (E)((u32)idx << C.esh);
 #pragma unroll
     for (int g = 0; g < NJ; ++g) {
       // lane masks from single compares, combined in scalar code
@@ -1017,7 +1032,7 @@ __device__ __forceinline__ int dp_emit_group(const DpEmitCell<D, E>& C, int k0) 
         // the running length enters as mbcnt's addend
         const u32 e = min(__builtin_amdgcn_mbcnt_hi((u32)(hm >> 32), __builtin_amdgcn_mbcnt_lo((u32)hm, cnt[g])),
                           last);
-        *(E*)(rows + (u64)((e << ESH) + (u32)g * rstepb)) = (E)idx;
+        *(E*)(rows + (u64)((e << ESH) + (u32)g * rstepb)) = ev;
       }
       cnt[g] += (u32)__popcll(hm);
     }
@@ -1025,9 +1040,9 @@ __device__ __forceinline__ int dp_emit_group(const DpEmitCell<D, E>& C, int k0) 
   int mx = 0;
 #pragma unroll
   for (int g = 0; g < NJ; ++g) {
-    // zero entries up to the pair kernels' chunk of the row's end
+    // padding entries up to the pair kernels' chunk of the row's end
     const u32 c0 = min(cnt[g], (u32)C.K);
-    if ((u32)lane < dp_row_pad<E>(c0)) *(E*)(rows + (u64)(((c0 + (u32)lane) << ESH) + (u32)g * rstepb)) = (E)0;
+    if ((u32)lane < dp_row_pad<E>(c0)) *(E*)(rows + (u64)(((c0 + (u32)lane) << ESH) + (u32)g * rstepb)) = (E)C.pv;
     if (lane == 0) {
       C.nlcnt[C.slot0 + k0 + NW * g] = (i32)cnt[g];
       if (cnt[g] > (u32)C.K) C.ctl->overflow = 1;
@@ -1101,7 +1116,7 @@ __device__ __forceinline__ bool dp_emit_in_box(const int4 q, const DpEmitBox& b)
 template <int D, typename E>
 __device__ __forceinline__ void dp_emit_particles_tab(const DpTileSmem<D>& S, const DpRep& P, const double* xref,
                                                       const int4* srq, u16* ctab, i32* ekeep, int K, E* nl,
-                                                      i32* nlcnt, DpCtl* ctl, int& st_count) {
+                                                      i32* nlcnt, DpCtl* ctl, int efmt, int& st_count) {
   constexpr int N3 = DpPow3<D>::value;
   constexpr int NROW = N3 / 3;
   constexpr int G = 8;
@@ -1187,7 +1202,8 @@ __device__ __forceinline__ void dp_emit_particles_tab(const DpTileSmem<D>& S, co
     for (int j0 = 0; j0 < npw; j0 += G) {
       const int nj = min(G, npw - j0);
       const int k0 = wave + NW * j0;
-      const DpEmitCell<D, E> C{S, P, xref, srq, ctab, nl, nlcnt, ctl, K, Ks, self0, slot0, ncand, cs0, lo16, hi16};
+      const DpEmitCell<D, E> C{S,     P,     xref, srq, ctab, nl,   nlcnt, ctl, K, Ks, self0, slot0, ncand, cs0,
+                               lo16,  hi16,  dp_efmt_esh(efmt), dp_efmt_pad(efmt, S.nrec)};
       int c = 0;
       switch (nj) {
         case 1: c = dp_emit_group<D, E, 1>(C, k0); break;
@@ -1213,11 +1229,13 @@ __device__ __forceinline__ void dp_emit_particles_tab(const DpTileSmem<D>& S, co
 // Lanes of one home cell read the same staged record at the same time.
 template <int D, typename E>
 __device__ __forceinline__ void dp_emit_lanes(const DpTileSmem<D>& S, const DpRep& P, const double* xref,
-                                              const int4* srq, int K, E* nl, i32* nlcnt, DpCtl* ctl,
+                                              const int4* srq, int K, E* nl, i32* nlcnt, DpCtl* ctl, int efmt,
                                               int& st_count) {
   constexpr int N3 = DpPow3<D>::value;
   constexpr int EB = 8 * (int)sizeof(E);    // bits per entry
   constexpr int EPW = 64 / EB;              // entries per 8-byte store
+  const int esh = dp_efmt_esh(efmt);
+  const u32 pv = dp_efmt_pad(efmt, S.nrec);
   const i64 Ks = (i64)((K + 7) & ~7);
   const int lo16 = P.nlq16_lo, hi16 = P.nlq16_hi;
   for (int k = S.k0 + threadIdx.x; k < S.k1; k += blockDim.x) {
@@ -1243,7 +1261,7 @@ __device__ __forceinline__ void dp_emit_lanes(const DpTileSmem<D>& S, const DpRe
         // the screen's slack (rare): the exact decision
         if (!h && s16 <= hi16 && j != self) h = dp_emit_exact<D>(S, P, xref, srq, self, j, slot);
         if (h) {
-          pend |= (u64)(u32)j << (EB * (cnt % EPW));
+          pend |= (u64)((u32)j << esh) << (EB * (cnt % EPW));
           if (cnt % EPW == EPW - 1) {
             if (cnt - (EPW - 1) < (u32)K) *(u64*)(row + (cnt - (EPW - 1))) = pend;
             pend = 0;
@@ -1252,10 +1270,13 @@ __device__ __forceinline__ void dp_emit_lanes(const DpTileSmem<D>& S, const DpRe
         }
       }
     }
-    // the last, partial group, its entries past cnt zero (the pad of
-    // dp_row_pad)
+    // the last, partial group, its entries past cnt the padding entry (the
+    // pad of dp_row_pad)
     const u32 g0 = cnt - cnt % EPW;
-    if (cnt % EPW != 0 && g0 < (u32)K) *(u64*)(row + g0) = pend;
+    if (cnt % EPW != 0 && g0 < (u32)K) {
+      for (u32 q = cnt % EPW; q < (u32)EPW; ++q) pend |= (u64)pv << (EB * q);
+      *(u64*)(row + g0) = pend;
+    }
     nlcnt[slot] = (i32)cnt;
     if (cnt > (u32)K) ctl->overflow = 1;
     st_count = max(st_count, (int)cnt);
@@ -1272,7 +1293,8 @@ __global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* 
                                                                const DpRep* __restrict__ reps, int R,
                                                                i64 ntiles, int K, int cap, E* nl, i32* nlcnt,
                                                                i32* tiletab, int tstride, DpCtl* ctl, int force,
-                                                               int lanes, const i32* __restrict__ todo) {
+                                                               int lanes, const i32* __restrict__ todo, int efmt,
+                                                               int cap_pair) {
   if (!force && !ctl->rebuild) return;
   // non-finite positions were binned into arbitrary cells: skip the (then
   // unbounded) emit; the error is reported after the step block
@@ -1295,7 +1317,14 @@ __global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* 
     const DpRep& P = reps[__builtin_amdgcn_readfirstlane(S.rep)];  // uniform: scalar loads
     const bool fits = S.fits != 0;
     dp_tile_store<D>(S, P, tiletab + t * (i64)tstride);
-    if (tid == 0) tiletab[t * (i64)tstride + 6] = fits ? 0 : 1;
+    // entry format of the tile: byte offsets (efmt) where the halo fits the
+    // pair kernels' LDS record area (byte offsets then stay below 48 KB);
+    // plain staged-halo indices otherwise
+    const int efmt_t = ((efmt & 31) != 0 && S.nrec + (efmt >> 8) <= cap_pair) ? efmt : 0;
+    if (tid == 0) {
+      tiletab[t * (i64)tstride + 6] = fits ? 0 : 1;
+      tiletab[t * (i64)tstride + 7] = efmt_t != 0 ? 1 : 0;
+    }
     if (fits) {
       for (int idx = tid; idx < S.nrec; idx += blockDim.x) {
         const int u = dp_halo_of<D>(S, P.nhalo, idx);
@@ -1303,17 +1332,19 @@ __global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* 
       }
     }
     if (tid == 0) {
-      // entries index the staged halo: E must hold nrec - 1
-      if (sizeof(E) < 4 && S.nrec > 65536) ctl->cell_overflow = 1;
+      // entries address the staged halo: E must hold the last entry (the
+      // sentinel's, nrec << esh, or (nrec - 1) << esh)
+      if (sizeof(E) < 4 && S.nrec > dp_efmt_hlim(efmt_t)) ctl->cell_overflow = 1;
       st_cell = max(st_cell, S.maxc);
       st_halo = max(st_halo, S.nrec);
     }
     __syncthreads();
     const bool cm = P.collapsed_mask != 0;
     if (fits) {
-      if (cm) dp_emit_particles<D, E, true, true>(S, P, xref, cellpack, qinv, srq, K, nl, nlcnt, ctl, st_count);
-      else if (lanes) dp_emit_lanes<D, E>(S, P, xref, srq, K, nl, nlcnt, ctl, st_count);
-      else dp_emit_particles_tab<D, E>(S, P, xref, srq, ctab, ekeep, K, nl, nlcnt, ctl, st_count);
+      if (cm)
+        dp_emit_particles<D, E, true, true>(S, P, xref, cellpack, qinv, srq, K, nl, nlcnt, ctl, efmt_t, st_count);
+      else if (lanes) dp_emit_lanes<D, E>(S, P, xref, srq, K, nl, nlcnt, ctl, efmt_t, st_count);
+      else dp_emit_particles_tab<D, E>(S, P, xref, srq, ctab, ekeep, K, nl, nlcnt, ctl, efmt_t, st_count);
     }
     __syncthreads();
   }
@@ -1376,7 +1407,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile_far(const double*
                                                                    const DpRep* __restrict__ reps, i64 ntiles, int K,
                                                                    E* nl, i32* nlcnt, const i32* tiletab, int tstride,
                                                                    DpCtl* ctl, int force,
-                                                                   const i32* __restrict__ todo) {
+                                                                   const i32* __restrict__ todo, int efmt) {
   if (!force && !ctl->rebuild) return;
   if (ctl->nonfinite) return;
   __shared__ DpTileSmem<D> S;
@@ -1387,6 +1418,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile_far(const double*
     const i32* tt = tiletab + t * (i64)tstride;
     if (tt[6] == 0) continue;  // uniform over the workgroup
     dp_tile_load<D>(S, tt, 0);
+    const int efmt_t = S.ebo ? efmt : 0;  // the tile's entry format (k_emit_tile's choice)
     const DpRep& P = reps[__builtin_amdgcn_readfirstlane(S.rep)];  // uniform: scalar loads
     // st_off is not part of the stored tables
     constexpr int n3 = DpPow3<D>::value;
@@ -1411,9 +1443,10 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile_far(const double*
     }
     __syncthreads();
     if (P.collapsed_mask != 0)
-      dp_emit_particles<D, E, false, true>(S, P, xref, cellpack, qinv, nullptr, K, nl, nlcnt, ctl, st_count);
+      dp_emit_particles<D, E, false, true>(S, P, xref, cellpack, qinv, nullptr, K, nl, nlcnt, ctl, efmt_t, st_count);
     else
-      dp_emit_particles<D, E, false, false>(S, P, xref, cellpack, qinv, nullptr, K, nl, nlcnt, ctl, st_count);
+      dp_emit_particles<D, E, false, false>(S, P, xref, cellpack, qinv, nullptr, K, nl, nlcnt, ctl, efmt_t,
+                                            st_count);
     __syncthreads();
   }
   int mx = st_count;
@@ -1452,10 +1485,12 @@ __global__ void k_from_orig(const T* src, T* dst, const i32* orig, i64 N, int W)
 template <int D, typename E>
 __global__ void __launch_bounds__(DP_TILE_THREADS) k_nl_export(const E* nl, const i32* nlcnt, const i32* orig,
                                                                const i32* cstart, const DpRep* reps, int R,
-                                                               i64 ntiles, int K, i32* out) {
+                                                               i64 ntiles, int K, int esh, const i32* tiletab,
+                                                               int tstride, i32* out) {
   __shared__ DpTileSmem<D> S;
   for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
     dp_tile_setup<D>(S, reps, R, cstart, t, 0x7fffffff);
+    const int sh = tiletab[t * (i64)tstride + 7] ? esh : 0;  // the tile's entry format
     const DpRep& P = reps[__builtin_amdgcn_readfirstlane(S.rep)];  // uniform: scalar loads
     for (int k = S.k0 + threadIdx.x; k < S.k1; k += blockDim.x) {
       const int m = dp_home_of<D>(S, P.nhome, k);
@@ -1466,7 +1501,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_nl_export(const E* nl, cons
       for (int kk = 0; kk < K; ++kk) {
         i32 val = -1;
         if (kk < cnt) {
-          const int idx = (int)nl[slot_i * (i64)((K + 7) & ~7) + kk];
+          const int idx = (int)((u32)nl[slot_i * (i64)((K + 7) & ~7) + kk] >> sh);
           const int u = dp_halo_of<D>(S, P.nhalo, idx);
           val = orig[S.hstart[u] + (idx - S.hbase[u])];
         }

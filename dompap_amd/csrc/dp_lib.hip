@@ -80,6 +80,19 @@ int rs_for_dim(int D) {
   return rs;
 }
 
+// Entry shift of the pair kernels' staged records: mixed precision, log2 of
+// the record's bytes (the coordinates, + the type with several types, in a
+// power of two of ints: DP_LS of the pair source), so that an entry is a
+// byte offset; FP64 mode, 0.
+int staged_esh(int D, int mode, int ntypes) {
+  if (mode != DP_MIXED) return 0;
+  int ls = 1;
+  while (ls < D + (ntypes > 1 ? 1 : 0)) ls *= 2;
+  int esh = 2;
+  while ((1 << esh) < 4 * ls) ++esh;
+  return esh;
+}
+
 template <typename T>
 void dfree(T*& p) {
   if (p) (void)hipFree((void*)p);
@@ -131,6 +144,7 @@ CodeCache& code_cache() {
 struct PairConfig {
   int D, mode, ntypes, even, rs, tpp, nlw, ob, pairprec;
   int cmask;  // dimensions with a single cell in some replica
+  int esh = 0, sent = 0;  // entry = staged index << esh; rows padded with the sentinel's entry
   int hmax = 0, omax = 0;  // LDS tile-table capacity: largest halo / home cell count of the plan (0: the limits)
 };
 
@@ -163,8 +177,9 @@ std::string pair_source(const PairConfig& c, const std::vector<std::string>& src
   char buf[512];
   snprintf(buf, sizeof(buf),
            "#define DP_D %d\n#define DP_MODE %d\n#define DP_NTYPES %d\n#define DP_NPOT %d\n#define DP_RS %d\n"
-           "#define DP_TPP %d\n#define DP_NLW %d\n#define DP_OB %d\n#define DP_PAIRPREC %d\n#define DP_CMASK %d\n",
-           c.D, c.mode, c.ntypes, (int)srcs.size(), c.rs, c.tpp, c.nlw, c.ob, c.pairprec, c.cmask);
+           "#define DP_TPP %d\n#define DP_NLW %d\n#define DP_OB %d\n#define DP_PAIRPREC %d\n#define DP_CMASK %d\n"
+           "#define DP_ESH %d\n#define DP_SENT %d\n",
+           c.D, c.mode, c.ntypes, (int)srcs.size(), c.rs, c.tpp, c.nlw, c.ob, c.pairprec, c.cmask, c.esh, c.sent);
   src += buf;
   if (c.hmax > 0 && c.omax > 0) {
     snprintf(buf, sizeof(buf), "#define DP_TS_HALO %d\n#define DP_TS_HOME %d\n", c.hmax, c.omax);
@@ -307,6 +322,7 @@ struct dp_ctx {
   size_t nl_bytes = 0;
   // tile / entry-format plan (refresh_reps)
   int nlw = 16, ob = 11, tpp = 8, qbits = 24, cap = 1024, cap_emit = 1024;
+  int efmt = 0;  // neighbour-entry format: esh | sent << 8 (dp_efmt_hlim)
   int emit_lanes = 0;  // 1: the emit takes one lane per home particle (cells of a few particles)
   int nlw_min = 16;    // 32 after a tile's halo outgrew 16-bit entries (kept for later builds)
   bool unit_mass = true;  // every mass is 1.0 (as at creation)
@@ -734,9 +750,44 @@ int refresh_reps(dp_ctx* ctx) {
       P.nlq16_hi = (i32)std::min(hi16, 2147483647.0);
     }
   }
-  // entry format: the index of a neighbour in its tile's staged halo; 16
-  // bits unless a halo could hold more than 2^16 particles
+  // entry format: the index of a neighbour in its tile's staged halo,
+  // shifted left by esh; in mixed precision esh = log2 of the staged record's
+  // bytes, so that an entry is the record's byte offset in the pair kernels'
+  // LDS record area.  Mixed-precision rows are padded with the entry of a
+  // sentinel record (staged after the halo, 2^31 fixed-point units from the
+  // home box's centre along every dimension) where that is provably beyond
+  // the cut of every home particle: along d no displacement from it is
+  // smaller than b_d = 2^31 - (T_d Wq_d / 2 + skin 2^q + 2) (a home
+  // particle's record lies within T_d Wq_d / 2 + skin 2^q of the box centre).
+  // Not with a single-cell dimension (its minimum image would fold the
+  // sentinel back).  The emit gives a tile this format when its halo fits
+  // the pair kernels' LDS record area (at most 48 KB of records, so byte
+  // offsets fit 16 bits), other tiles plain indices: 16-bit entries unless a
+  // halo could exceed 2^16 records.
   (void)occ_max;
+  int esh = 0, sent = 0;
+  if (ctx->mode == DP_MIXED) {
+    esh = staged_esh(D, ctx->mode, ctx->ntypes);
+    sent = (ctx->cmask == 0) ? 1 : 0;
+    for (int r = 0; r < ctx->R && sent; ++r) {
+      const DpRep& P = reps[r];
+      const double cut = ctx->cutoff[r], sk = std::max(ctx->skin[r], 0.0);
+      if (!std::isfinite(cut)) {
+        sent = 0;
+        break;
+      }
+      long double s2 = 0.0L;
+      for (int d = 0; d < D; ++d) {
+        const long double b = 2147483648.0L - ((long double)P.T[d] * (long double)P.Wq[d] / 2.0L +
+                                               (long double)sk * (long double)qs + 2.0L);
+        if (!(b > 0.0L)) sent = 0;
+        s2 += b * b;
+      }
+      const long double need = ((long double)cut + 2.0L * (long double)sk) * (long double)qs;
+      if (!(s2 > need * need * 1.0001L + 1048576.0L)) sent = 0;
+    }
+  }
+  const int efmt = esh | (sent << 8);
   int nlw = ctx->nlw_min;
   if (halo_occ_max * 4.0 + 64.0 >= 65536.0) nlw = 32;
   const int ob = nlw;
@@ -752,12 +803,14 @@ int refresh_reps(dp_ctx* ctx) {
   }
   tiles_sig.push_back(nlw);
   tiles_sig.push_back(qbits);
+  tiles_sig.push_back(efmt);
   const bool cells_changed = cells_sig != ctx->cells_sig;
   const bool tiles_changed = tiles_sig != ctx->tiles_sig;
   ctx->cells_sig = cells_sig;
   ctx->tiles_sig = tiles_sig;
   ctx->nlw = nlw;
   ctx->ob = ob;
+  ctx->efmt = efmt;
   ctx->tpp = tpp;
   ctx->halo_occ = halo_occ_max;
   ctx->rec_bytes = rec_bytes;
@@ -820,7 +873,10 @@ int refresh_reps(dp_ctx* ctx) {
   if (nl_bytes != ctx->nl_bytes || !ctx->nl) {
     dfree(ctx->nl);
     ctx->nl_bytes = 0;
-    if (hipMalloc(&ctx->nl, nl_bytes) != hipSuccess) {
+    // + DP_NL_SLACK: the pair kernels load a row's next 16-byte group before
+    // testing whether the row goes on (at most 8 x lanes-per-particle
+    // entries past its end; the last row's loads end in this slack)
+    if (hipMalloc(&ctx->nl, nl_bytes + DP_NL_SLACK) != hipSuccess) {
       ctx->nl = nullptr;
       return ctx->fail(DP_ERR_HIP, "cannot allocate neighbour list of %zu bytes", nl_bytes);
     }
@@ -928,7 +984,8 @@ int refresh_tab(dp_ctx* ctx) {
 // Compile (or fetch) the pair module for the current plan.
 int ensure_module(dp_ctx* ctx) {
   if (!ctx->have_pot) return ctx->fail(DP_ERR_STATE, "no potential loaded (dp_load_potential)");
-  PairConfig c{ctx->D, ctx->mode, ctx->ntypes, 0, ctx->rs, ctx->tpp, ctx->nlw, ctx->ob, ctx->pair_prec, ctx->cmask};
+  PairConfig c{ctx->D,  ctx->mode,      ctx->ntypes, 0, ctx->rs, ctx->tpp, ctx->nlw, ctx->ob, ctx->pair_prec,
+               ctx->cmask, ctx->efmt & 31, ctx->efmt >> 8};
   // tile tables in LDS sized for the plan's largest tile (rounded up to 16 cells)
   int hmax = 1, omax = 1;
   for (const auto& P : ctx->reps_h) {
@@ -1038,10 +1095,11 @@ int launch_emit_te(dp_ctx* ctx, int force) {
   const i32* todo = listed ? ctx->todo : nullptr;
   k_emit_tile<D, E><<<grid, DP_TILE_THREADS, ctx->lds_emit, ctx->st>>>(
       ctx->xref, ctx->cellpack, qinv, ctx->cstart, ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, ctx->cap_emit,
-      (E*)ctx->nl, ctx->nlcnt, ctx->tiletab, ctx->tstride, ctx->ctl_d, force, ctx->emit_lanes, todo);
+      (E*)ctx->nl, ctx->nlcnt, ctx->tiletab, ctx->tstride, ctx->ctl_d, force, ctx->emit_lanes, todo, ctx->efmt,
+      ctx->cap);
   k_emit_tile_far<D, E><<<grid, DP_TILE_THREADS, 0, ctx->st>>>(
       ctx->xref, ctx->cellpack, qinv, ctx->reps_d, ctx->ntiles, ctx->K, (E*)ctx->nl, ctx->nlcnt, ctx->tiletab,
-      ctx->tstride, ctx->ctl_d, force, todo);
+      ctx->tstride, ctx->ctl_d, force, todo, ctx->efmt);
   HIPCHK(hipGetLastError());
   return DP_OK;
 }
@@ -1210,7 +1268,7 @@ int check_ctl(dp_ctx* ctx) {
       return ctx->fail(DP_ERR_OVERFLOW,
                        "a tile's halo holds %d particles, more than the %d that %d-bit neighbour entries address; "
                        "the next build uses 32-bit entries",
-                       ctx->ctl_h->max_halo, 1 << 16, ctx->nlw);
+                       ctx->ctl_h->max_halo, dp_efmt_hlim(0), ctx->nlw);
     }
     return ctx->fail(DP_ERR_NONFINITE, "non-finite particle position encountered");
   }
@@ -1402,10 +1460,13 @@ int check_fold(const char* src, int precision, int pair_precision) {
   return check_folds(std::vector<std::string>{src}, precision, pair_precision)[0];
 }
 
-// Default plan used by the device-free compile checks.
+// Default plan used by the device-free compile checks (mixed precision: byte-
+// offset entries padded with the sentinel's, as in the usual plan).
 PairConfig default_config(int dim, int precision, int ntypes, int even, int pair_precision) {
   PairConfig c{dim, precision, ntypes, even, rs_for_dim(dim), 8, 16, 16,
                pair_precision ? 32 : 64, 0};
+  c.esh = staged_esh(dim, precision, ntypes);
+  c.sent = precision == DP_MIXED ? 1 : 0;
   return c;
 }
 
@@ -2071,11 +2132,11 @@ DP_API int dp_get_plan(dp_ctx* ctx, int32_t* out, int n) {
   if (!ctx || !out) return DP_ERR_ARG;
   (void)hipSetDevice(ctx->dev);
   CHK(refresh_reps(ctx));
-  const int32_t v[14] = {ctx->tpp,  ctx->nlw, ctx->ob, ctx->qbits, ctx->cap, ctx->cap_emit,
+  const int32_t v[16] = {ctx->tpp,  ctx->nlw, ctx->ob, ctx->qbits, ctx->cap, ctx->cap_emit,
                          (int32_t)ctx->ntiles, ctx->tile_grid, ctx->fused, ctx->reps_h[0].nhome,
                          ctx->reps_h[0].nhalo, (int32_t)ctx->Ctot, ctx->graph_block,
-                         ctx->graph_events_ok ? 1 : 0};
-  for (int k = 0; k < n && k < 14; ++k) out[k] = v[k];
+                         ctx->graph_events_ok ? 1 : 0, ctx->efmt & 31, ctx->efmt >> 8};
+  for (int k = 0; k < n && k < 16; ++k) out[k] = v[k];
   return DP_OK;
 }
 
@@ -2107,10 +2168,12 @@ int launch_export_t(dp_ctx* ctx, i32* out) {
   const int g = ctx->tile_grid;
   if (ctx->nlw == 16)
     k_nl_export<D, u16><<<g, DP_TILE_THREADS, 0, ctx->st>>>((const u16*)ctx->nl, ctx->nlcnt, ctx->orig, ctx->cstart,
-                                                            ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, out);
+                                                            ctx->reps_d, ctx->R, ctx->ntiles, ctx->K,
+                                                            ctx->efmt & 31, ctx->tiletab, ctx->tstride, out);
   else
     k_nl_export<D, u32><<<g, DP_TILE_THREADS, 0, ctx->st>>>((const u32*)ctx->nl, ctx->nlcnt, ctx->orig, ctx->cstart,
-                                                            ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, out);
+                                                            ctx->reps_d, ctx->R, ctx->ntiles, ctx->K,
+                                                            ctx->efmt & 31, ctx->tiletab, ctx->tstride, out);
   HIPCHK(hipGetLastError());
   return DP_OK;
 }

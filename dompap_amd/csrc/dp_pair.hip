@@ -13,6 +13,10 @@
 //   #define DP_PAIRPREC <32|64> mixed mode: precision of the in-range pair terms
 //   #define DP_CMASK <mask>     dimensions with a single cell (minimum image in
 //                               the pair loop); 0 in the usual case
+//   #define DP_ESH <n>          entry = staged index << DP_ESH (mixed mode: the
+//                               byte offset of the staged record; 0 in FP64 mode)
+//   #define DP_SENT <0|1>       mixed mode: rows are padded with the entry of a
+//                               sentinel record staged after the halo
 // and, per potential k, namespace dp_p<k> holding DP_EVEN (1 if f(x)/x and
 // u(x) are functions of x^2) and the SymPy-generated device functions
 //   template<class T> T dp_pot_u(T x);   u(x)
@@ -40,6 +44,12 @@
 #endif
 #ifndef DP_CMASK
 #define DP_CMASK 0
+#endif
+#ifndef DP_ESH
+#define DP_ESH 0
+#endif
+#ifndef DP_SENT
+#define DP_SENT 0
 #endif
 
 #define DP_N3 (DpPow3<DP_D>::value)
@@ -90,12 +100,9 @@ __device__ __forceinline__ i64 dp_row_stride(int K) { return (i64)((K + 7) & ~7)
 #define DP_EPL (8 / (int)sizeof(dp_ent_t))  // entries per 8-byte load
 
 // Row walking: each lane loads DP_CPL chunks of DP_EPL entries (8 bytes per
-// chunk) at a time and keeps DP_PF such loads in flight, so that the latency
-// of the row's loads (the row is streamed from HBM once per step) overlaps
-// the pair arithmetic of the chunks before them.
-#ifndef DP_PF
-#define DP_PF 1
-#endif
+// chunk) at a time, one such load ahead of the chunks being evaluated, so
+// that the latency of the row's loads (the row is streamed from HBM once per
+// step) overlaps the pair arithmetic of the chunks before them.
 #ifndef DP_CPL
 #define DP_CPL 2
 #endif
@@ -114,44 +121,50 @@ __device__ __forceinline__ DpLoad dp_ld_entries(const dp_ent_t* p) {
   return v;
 }
 
+// The chunks of one load at row entry kj.
+template <class Fn>
+__device__ __forceinline__ void dp_load_chunks(const DpLoad& cur, int kj, int cnt, Fn& fn) {
+#pragma unroll
+  for (int j = 0; j < DP_CPL; ++j, kj += DP_EPL) {
+    if (j > 0 && kj >= cnt) break;
+    int idx[DP_EPL];
+#if DP_NLW == 16
+    idx[0] = (int)(cur.w[2 * j] & 0xffffu);
+    idx[1] = (int)(cur.w[2 * j] >> 16);
+    idx[2] = (int)(cur.w[2 * j + 1] & 0xffffu);
+    idx[3] = (int)(cur.w[2 * j + 1] >> 16);
+#else
+    idx[0] = (int)cur.w[2 * j];
+    idx[1] = (int)cur.w[2 * j + 1];
+#endif
+    fn(idx, min(DP_EPL, cnt - kj));
+  }
+}
+
 // Call fn(idx, nvalid) for the chunks of a row of length cnt that belong to
 // lane `sub` of its group of tpp lanes: loads of DP_CPL chunks at entries
 // [W * (sub + j * tpp), + W), W = DP_EPL * DP_CPL, for j = 0, 1, ...; idx[c]
 // for c >= nvalid is 0: the emit pads every row with zero entries up to a
-// multiple of DP_EPL, and a row's last chunk ends within that padding.
+// multiple of DP_EPL, and a row's last chunk ends within that padding.  The
+// next load is issued before the current chunks are evaluated, whether or not
+// the row goes on (it then reads the next rows or the list's tail slack,
+// DP_NL_SLACK), and the two loads alternate between two register sets.
 template <class Fn>
 __device__ __forceinline__ void dp_for_chunks(const dp_ent_t* row, int cnt, int sub, int tpp, Fn& fn) {
   constexpr int W = DP_EPL * DP_CPL;
   const int step = W * tpp;
   int k0 = sub * W;
-  DpLoad buf[DP_PF];
-#pragma unroll
-  for (int p = 0; p < DP_PF; ++p) {
-#pragma unroll
-    for (int q = 0; q < 2 * DP_CPL; ++q) buf[p].w[q] = 0u;
-    if (k0 + p * step < cnt) buf[p] = dp_ld_entries(row + k0 + p * step);
-  }
-  for (; k0 < cnt; k0 += step) {
-    const DpLoad cur = buf[0];
-#pragma unroll
-    for (int p = 0; p + 1 < DP_PF; ++p) buf[p] = buf[p + 1];
-    if (k0 + DP_PF * step < cnt) buf[DP_PF - 1] = dp_ld_entries(row + k0 + DP_PF * step);
-#pragma unroll
-    for (int j = 0; j < DP_CPL; ++j) {
-      const int kj = k0 + j * DP_EPL;
-      if (j > 0 && kj >= cnt) break;
-      int idx[DP_EPL];
-#if DP_NLW == 16
-      idx[0] = (int)(cur.w[2 * j] & 0xffffu);
-      idx[1] = (int)(cur.w[2 * j] >> 16);
-      idx[2] = (int)(cur.w[2 * j + 1] & 0xffffu);
-      idx[3] = (int)(cur.w[2 * j + 1] >> 16);
-#else
-      idx[0] = (int)cur.w[2 * j];
-      idx[1] = (int)cur.w[2 * j + 1];
-#endif
-      fn(idx, min(DP_EPL, cnt - kj));
-    }
+  if (k0 >= cnt) return;
+  DpLoad a = dp_ld_entries(row + k0);
+  for (;;) {
+    const DpLoad b = dp_ld_entries(row + k0 + step);
+    dp_load_chunks(a, k0, cnt, fn);
+    k0 += step;
+    if (k0 >= cnt) break;
+    a = dp_ld_entries(row + k0 + step);
+    dp_load_chunks(b, k0, cnt, fn);
+    k0 += step;
+    if (k0 >= cnt) break;
   }
 }
 
@@ -247,6 +260,11 @@ __device__ __forceinline__ void dp_st_rec(i32* p, const DpRec& r) {
 // halves the record for D = 2 and D = 4 (the HBM record keeps its type slot).
 #define DP_LS ((DP_D + (DP_NTYPES > 1 ? 1 : 0)) <= 1 ? 1 : (DP_D + (DP_NTYPES > 1 ? 1 : 0)) <= 2 ? 2 : (DP_D + (DP_NTYPES > 1 ? 1 : 0)) <= 4 ? 4 : 8)
 static_assert(DP_LS <= DP_RS, "staged record larger than the HBM record");
+#if DP_MODE == 1
+static_assert(4 * DP_LS == (1 << DP_ESH), "mixed-mode entries are byte offsets of staged records");
+#else
+static_assert(DP_ESH == 0 && DP_SENT == 0, "FP64-mode entries are staged indices, padded with 0");
+#endif
 // First DP_LS ints of a record (the remaining slots read as 0).
 __device__ __forceinline__ DpRec dp_ld_lrec(const i32* p) {
   DpRec r;
@@ -431,12 +449,13 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
       DpRec rj[DP_EPL];
 #pragma unroll
       for (int c = 0; c < DP_EPL; ++c) {
-        // entries past the row's end are the emit's zero padding (staged
-        // index 0), read like the others and excluded from the sums below
-        const int ic = idx[c];
+        // entries past the row's end are the emit's padding (the sentinel's
+        // entry, or 0), read like the others and excluded from the sums
+        // below: by the sentinel's distance, or by their position
         if (LDS) {
-          rj[c] = dp_ld_lrec(srec + ic * DP_LS);
+          rj[c] = dp_ld_lrec((const i32*)((const char*)srec + idx[c]));  // entry = byte offset
         } else {
+          const int ic = (c < nvalid) ? (S.ebo ? (idx[c] >> DP_ESH) : idx[c]) : 0;
           const int u = dp_halo_of<DP_D>(S, nh, ic);
           rj[c] = dp_halo_rec(a, S, P, u, (i64)S.hstart[u] + (ic - S.hbase[u]));
         }
@@ -466,7 +485,8 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
         tt[c] = dp_tt(ti, (DP_NTYPES > 1) ? rj[c].v[DP_D] : 0);
         const DpPC& pc = DP_PC(tt[c]);
         // the guards bound the FP32-rounded value; the FP64 one lies closer
-        in[c] = (c < nvalid) && (sd[c] < (double)pc.ghi);  // else certainly beyond the cut
+        // (a padding entry: beyond ghi when it names the sentinel)
+        in[c] = (DP_SENT && LDS ? true : (c < nvalid)) && (sd[c] < (double)pc.ghi);  // else certainly beyond the cut
         nr[c] = in[c] && (sd[c] > (double)pc.glo);
         near |= nr[c];
       }
@@ -476,7 +496,7 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
 #pragma unroll
         for (int c = 0; c < DP_EPL; ++c) {
           if (!nr[c]) continue;
-          const i64 slot_j = dp_slot_of(S, nh, idx[c]);
+          const i64 slot_j = dp_slot_of(S, nh, idx[c] >> DP_ESH);
           double s = 0.0;
 #pragma unroll
           for (int d = 0; d < DP_D; ++d) {
@@ -495,7 +515,16 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
         // too and their term is selected away (no trap on inf / NaN)
 #if DP_FOLD
         // eps f(x)/r per fixed-point unit straight from the squared
-        // fixed-point distance: the scales are in the coefficients
+        // fixed-point distance: the scales are in the coefficients; summed
+        // only where the entry is in range (no zero term for the others)
+        if (KIND != 1) {
+          if (in[c]) {
+            const double fs = dp_pot_gsk<double>(sd[c], pc.gk);
+#pragma unroll
+            for (int d = 0; d < DP_D; ++d) f[d] = fma(fs, dq[c][d], f[d]);
+          }
+          continue;
+        }
         const double fs = in[c] ? dp_pot_gsk<double>(sd[c], pc.gk) : 0.0;
 #else
         const double sx = sd[c] * pc.ks;
@@ -743,7 +772,9 @@ __device__ void dp_tile_run(const DpPairArgs& a) {
   for (i64 j = 0;; ++j) {
     const i64 t = dp_tile_of(j, a.ntiles);
     if (t >= cend) break;
-    dp_tile_load<DP_D>(S, a.tiletab + t * (i64)a.tstride, a.cap);
+    // (mixed mode: the LDS path reads records by byte-offset entries; with a
+    // sentinel, one record of the area is the sentinel's)
+    dp_tile_load<DP_D>(S, a.tiletab + t * (i64)a.tstride, a.cap - DP_SENT, DP_ESH != 0);
     const DpRep& P = a.reps[__builtin_amdgcn_readfirstlane(S.rep)];  // uniform: scalar loads
     const int nh = P.nhalo;
     const bool fits = S.fits != 0;
@@ -831,6 +862,23 @@ __device__ void dp_tile_run(const DpPairArgs& a) {
         for (int d = 0; d < DP_D; ++d) sxd[idx * DP_D + d] = t[d];
         if (DP_NTYPES > 1) stype[idx] = a.type[slot];
       }
+    }
+#endif
+#if DP_MODE == 1 && DP_SENT
+    if (fits && tid == 0) {
+      // the sentinel record (staged index nrec, the rows' padding entry):
+      // 2^31 fixed-point units from the home box's centre along every
+      // dimension, so that its exact integer displacement from any home
+      // particle lies beyond the cut (the host checks this bound per plan)
+      DpRec r;
+#pragma unroll
+      for (int q = 0; q < DP_RS; ++q) r.v[q] = 0;
+#pragma unroll
+      for (int d = 0; d < DP_D; ++d) {
+        const u32 c = (P.nc[d] == 1) ? 0u : (u32)(((i64)(P.T[d] - 1) * (i64)P.Wq[d]) / 2);
+        r.v[d] = (i32)(c + 0x80000000u);
+      }
+      dp_st_lrec(stage_i + S.nrec * DP_LS, r);
     }
 #endif
 #if DP_NTYPES > 1

@@ -158,7 +158,7 @@ class Engine:
 
     PLAN_KEYS = ("threads_per_particle", "entry_bits", "offset_bits", "fixed_point_bits", "lds_records",
                  "lds_records_emit", "tiles", "tile_grid", "fused", "home_cells_per_tile", "halo_cells_per_tile",
-                 "cells", "steps_per_block", "graph_events")
+                 "cells", "steps_per_block", "graph_events", "entry_shift", "sentinel_padding")
 
     def plan(self) -> dict:
         out = np.zeros(len(self.PLAN_KEYS), dtype=np.int32)

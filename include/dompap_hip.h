@@ -159,7 +159,9 @@ int dp_set_cell_scale(dp_ctx* ctx, double scale);
 /* Current plan: tpp, entry bits, offset bits, fixed-point bits, LDS record
  * capacity (pair, emit), tiles, tile grid, fused, home / halo cells per tile
  * of replica 0, cells, steps per step block, whether timed step blocks record
- * their events inside the graph (first n values). */
+ * their events inside the graph, entry shift (an entry is a staged-halo index
+ * shifted left by it), whether rows are padded with a sentinel record's entry
+ * (first n values). */
 int dp_get_plan(dp_ctx* ctx, int32_t* out, int n);
 
 /* RNG key and step counter. */
@@ -172,8 +174,10 @@ int dp_set_replica_seeds(dp_ctx* ctx, const uint64_t* seeds);
 
 /* Neighbour list in the reference layout int32 [N][K], -1 padded
  * (Simulation.neighbor_list); rows in stencil order, members in ascending id.
- * (On the device the list is kept as 16- or 32-bit (stencil cell, offset in
- * cell) entries over cell-sorted particles.) */
+ * (On the device the list is kept as 16- or 32-bit entries over cell-sorted
+ * particles: the index of the neighbour in its tile's staged halo, shifted
+ * left so that in mixed precision it is the byte offset of the neighbour's
+ * staged record.) */
 int dp_get_neighbor_list(dp_ctx* ctx, int32_t* out);
 /* Row lengths int32 [N]: get_number_of_neighbors (neighbor_list.py:204-207). */
 int dp_get_neighbor_counts(dp_ctx* ctx, int32_t* out);
