@@ -104,7 +104,8 @@ struct DpCtl {
   i32 blk_done;         // steps the last step block completed
   i32 blk_perstep;      // 1: the block rebuilds (gated) before every step and never stops early
   i32 emit_ntodo;       // tiles in the emit's tile list (batches: the rebuilding replicas' tiles)
-  i32 pad_[2];
+  i32 far_tiles;        // the rebuild's emit marked a tile whose halo exceeds its LDS record area
+  i32 pad_[1];
   // step blocks: the rebuild pipeline runs once at the block's start; step p
   // of the block runs while no earlier step of the block raised blk_need and
   // blk_base + p < blk_target (blk_perstep = 0); or the pipeline runs before

@@ -214,6 +214,7 @@ template <int D>
 __global__ void k_bin(const double* x, const i32* rep, const DpRep* reps, i64 N, i32* cell_of, i32* rank,
                       i32* count, DpCtl* ctl, const i32* rb) {
   if (!ctl->rebuild) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->far_tiles = 0;  // set again by this rebuild's emit
   const int lane = threadIdx.x & 63;
   for (i64 base = (i64)blockIdx.x * blockDim.x; base < N; base += (i64)gridDim.x * blockDim.x) {
     const i64 i = base + threadIdx.x;
@@ -1324,6 +1325,7 @@ __global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* 
     if (tid == 0) {
       tiletab[t * (i64)tstride + 6] = fits ? 0 : 1;
       tiletab[t * (i64)tstride + 7] = efmt_t != 0 ? 1 : 0;
+      if (!fits) ctl->far_tiles = 1;  // plain store: k_emit_tile_far has work
     }
     if (fits) {
       for (int idx = tid; idx < S.nrec; idx += blockDim.x) {
@@ -1410,6 +1412,9 @@ __global__ void __launch_bounds__(DP_TILE_THREADS) k_emit_tile_far(const double*
                                                                    const i32* __restrict__ todo, int efmt) {
   if (!force && !ctl->rebuild) return;
   if (ctl->nonfinite) return;
+  // a rebuild whose emit marked no tile (k_bin cleared the flag before it);
+  // an emit outside a rebuild (force) always looks at every tile
+  if (!force && ctl->far_tiles == 0) return;
   __shared__ DpTileSmem<D> S;
   int st_count = 0;
   const i64 ntodo = todo ? (i64)ctl->emit_ntodo : ntiles;

@@ -1097,7 +1097,8 @@ int launch_emit_te(dp_ctx* ctx, int force) {
       ctx->xref, ctx->cellpack, qinv, ctx->cstart, ctx->reps_d, ctx->R, ctx->ntiles, ctx->K, ctx->cap_emit,
       (E*)ctx->nl, ctx->nlcnt, ctx->tiletab, ctx->tstride, ctx->ctl_d, force, ctx->emit_lanes, todo, ctx->efmt,
       ctx->cap);
-  k_emit_tile_far<D, E><<<grid, DP_TILE_THREADS, 0, ctx->st>>>(
+  // (a bounded grid: the kernel usually finds no marked tile)
+  k_emit_tile_far<D, E><<<std::min(grid, 1024), DP_TILE_THREADS, 0, ctx->st>>>(
       ctx->xref, ctx->cellpack, qinv, ctx->reps_d, ctx->ntiles, ctx->K, (E*)ctx->nl, ctx->nlcnt, ctx->tiletab,
       ctx->tstride, ctx->ctl_d, force, todo, ctx->efmt);
   HIPCHK(hipGetLastError());
