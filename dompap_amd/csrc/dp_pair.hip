@@ -148,8 +148,10 @@ __device__ __forceinline__ void dp_load_chunks(const DpLoad& cur, int kj, int cn
 // multiple of DP_EPL, and a row's last chunk ends within that padding.  The
 // next load is issued before the current chunks are evaluated, whether or not
 // the row goes on (it then reads the next rows or the list's tail slack,
-// DP_NL_SLACK), and the two loads alternate between two register sets.
-template <class Fn>
+// DP_NL_SLACK).  With PP the two loads alternate between two register sets
+// (no copies, two instances of fn's code: for the hot loop only; in the other
+// instances the code would outgrow what the compiler inlines into the kernel).
+template <bool PP, class Fn>
 __device__ __forceinline__ void dp_for_chunks(const dp_ent_t* row, int cnt, int sub, int tpp, Fn& fn) {
   constexpr int W = DP_EPL * DP_CPL;
   const int step = W * tpp;
@@ -161,10 +163,14 @@ __device__ __forceinline__ void dp_for_chunks(const dp_ent_t* row, int cnt, int 
     dp_load_chunks(a, k0, cnt, fn);
     k0 += step;
     if (k0 >= cnt) break;
-    a = dp_ld_entries(row + k0 + step);
-    dp_load_chunks(b, k0, cnt, fn);
-    k0 += step;
-    if (k0 >= cnt) break;
+    if (PP) {
+      a = dp_ld_entries(row + k0 + step);
+      dp_load_chunks(b, k0, cnt, fn);
+      k0 += step;
+      if (k0 >= cnt) break;
+    } else {
+      a = b;
+    }
   }
 }
 
@@ -579,7 +585,7 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
 #endif
       }
     };
-    dp_for_chunks(row, cnt, sub, tpp, chunk);
+    dp_for_chunks<LDS && CM == 0 && KIND == 0>(row, cnt, sub, tpp, chunk);
 #else
     // ---------------- FP64 parity mode ----------------
     double xi_[DP_D];
@@ -632,7 +638,7 @@ __device__ __forceinline__ void dp_tile_particles(const DpPairArgs& a, const DpT
         }
       }
     };
-    dp_for_chunks(row, cnt, sub, tpp, chunk);
+    dp_for_chunks<LDS && CM == 0 && KIND == 0>(row, cnt, sub, tpp, chunk);
 #endif
     // combine the partial sums of the particle's DP_TPP lanes (every lane
     // of the group ends with the totals)
