@@ -105,7 +105,9 @@ struct DpCtl {
   i32 blk_perstep;      // 1: the block rebuilds (gated) before every step and never stops early
   i32 emit_ntodo;       // tiles in the emit's tile list (batches: the rebuilding replicas' tiles)
   i32 far_tiles;        // the rebuild's emit marked a tile whose halo exceeds its LDS record area
-  i32 pad_[1];
+  i32 pblk_n;           // batches: 256-particle blocks of the rebuilding replicas (k_rebuild_list)
+  i32 rb_n;             //   and the number of rebuilding replicas
+  i32 pad_[3];
   // step blocks: the rebuild pipeline runs once at the block's start; step p
   // of the block runs while no earlier step of the block raised blk_need and
   // blk_base + p < blk_target (blk_perstep = 0); or the pipeline runs before

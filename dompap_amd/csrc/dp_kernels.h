@@ -18,6 +18,28 @@ template <int D> struct DpPack {
 // that a gated (skipped) launch costs one short wave of workgroups.
 #define DP_FOR(i, n) for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (i64)gridDim.x * blockDim.x)
 
+// The rebuild pipeline's particles: with plist (batches), the ctl->pblk_n
+// blocks [start, end) of the replicas that rebuild in this step (DP_BLOCK
+// particles each, built by k_rebuild_list); else every particle (the kernels
+// then skip those of replicas that keep their list).  f(i, valid) is called by
+// every lane of a workgroup together (valid lanes form a prefix).
+template <class F>
+__device__ __forceinline__ void dp_for_rebuild(const int2* plist, const DpCtl* ctl, i64 N, F&& f) {
+  if (plist) {
+    const int nb = ctl->pblk_n;
+    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+      const int2 be = plist[b];
+      const i64 i = (i64)be.x + threadIdx.x;
+      f(i, i < (i64)be.y);
+    }
+  } else {
+    for (i64 base = (i64)blockIdx.x * blockDim.x; base < N; base += (i64)gridDim.x * blockDim.x) {
+      const i64 i = base + threadIdx.x;
+      f(i, i < N);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Control: decide which replicas rebuild this step (neighbor_list.py:183-201:
 // the previous integrator pass raised need[r] iff some particle of replica r
@@ -212,15 +234,13 @@ __device__ __forceinline__ void dp_stD(double* p, const double (&v)[D]) {
 // pipeline kernels leave its particles, records and list untouched.
 template <int D>
 __global__ void k_bin(const double* x, const i32* rep, const DpRep* reps, i64 N, i32* cell_of, i32* rank,
-                      i32* count, DpCtl* ctl, const i32* rb) {
+                      i32* count, DpCtl* ctl, const i32* rb, const int2* plist) {
   if (!ctl->rebuild) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl->far_tiles = 0;  // set again by this rebuild's emit
   const int lane = threadIdx.x & 63;
-  for (i64 base = (i64)blockIdx.x * blockDim.x; base < N; base += (i64)gridDim.x * blockDim.x) {
-    const i64 i = base + threadIdx.x;
-    const bool valid = i < N;  // valid lanes form a prefix of the wavefront
+  dp_for_rebuild(plist, ctl, N, [&](const i64 i, const bool valid) {  // valid lanes form a prefix of the wavefront
     i32 cid = -1;
-    if (valid && !rb[rep[i]]) {
+    if (valid && !plist && !rb[rep[i]]) {
       cid = cell_of[i];
     } else if (valid) {
       const DpRep& R = reps[rep[i]];
@@ -260,7 +280,7 @@ __global__ void k_bin(const double* x, const i32* rep, const DpRep* reps, i64 N,
     }
     b = __shfl(b, start);
     if (valid) rank[i] = b + (lane - start);
-  }
+  });
 }
 
 // Single-workgroup exclusive scan of the cell counts: blocks of 16384 cells,
@@ -470,24 +490,24 @@ __global__ void __launch_bounds__(DP_SCAN_THREADS) k_scan_apply(i32* count, i64 
 // Particle i to position cstart[cell] + rank in the cell-grouped order, with
 // its original id alongside (keys[pos]) for the in-cell sort.
 __global__ void k_scatter(const i32* cell_of, const i32* rank, const i32* cstart, const i32* orig, i64 N, i32* tmp,
-                          i32* keys, const DpCtl* ctl, const i32* rep, const i32* rb) {
+                          i32* keys, const DpCtl* ctl, const i32* rep, const i32* rb, const int2* plist) {
   if (!ctl->rebuild) return;
-  DP_FOR(i, N) {
-    if (!rb[rep[i]]) continue;
+  dp_for_rebuild(plist, ctl, N, [&](const i64 i, const bool valid) {
+    if (!valid || (!plist && !rb[rep[i]])) return;
     const i32 pos = cstart[cell_of[i]] + rank[i];
     tmp[pos] = (i32)i;
     keys[pos] = orig[i];
-  }
+  });
 }
 
 // In-cell order by original particle id (deterministic regardless of the
 // atomic ranks): the new position of a member is the number of members of
 // its cell with a smaller id, counted over the cell's contiguous keys.
 __global__ void k_cellsort(const i32* tmp, const i32* keys, const i32* cell_of, const i32* cstart, i64 N, i32* perm,
-                           const DpCtl* ctl, const i32* rep, const i32* rb) {
+                           const DpCtl* ctl, const i32* rep, const i32* rb, const int2* plist) {
   if (!ctl->rebuild) return;
-  DP_FOR(q, N) {
-    if (!rb[rep[q]]) continue;  // replicas own fixed slot ranges
+  dp_for_rebuild(plist, ctl, N, [&](const i64 q, const bool valid) {
+    if (!valid || (!plist && !rb[rep[q]])) return;  // replicas own fixed slot ranges
     const i32 p = tmp[q];
     const i32 c = cell_of[p];
     const i32 lo = cstart[c], hi = cstart[c + 1];
@@ -503,7 +523,7 @@ __global__ void k_cellsort(const i32* tmp, const i32* keys, const i32* cell_of, 
     }
     for (; k < hi; ++k) rnk += (keys[k] < key) ? 1 : 0;
     perm[lo + rnk] = p;
-  }
+  });
 }
 
 // Gather every per-particle array into the new order (into scratch), then
@@ -512,7 +532,8 @@ __global__ void k_cellsort(const i32* tmp, const i32* keys, const i32* cell_of, 
 template <int D>
 __global__ void k_permute(const i32* perm, i64 N, const double* x, const double* v, const double* beta,
                           const double* mass, const i32* img, const i32* type, const i32* orig,
-                          const i32* cell_of, char* scratch, const DpCtl* ctl, const i32* rep, const i32* rb) {
+                          const i32* cell_of, char* scratch, const DpCtl* ctl, const i32* rep, const i32* rb,
+                          const int2* plist) {
   if (!ctl->rebuild) return;
   double* sx = (double*)scratch;
   double* sv = sx + N * D;
@@ -522,8 +543,8 @@ __global__ void k_permute(const i32* perm, i64 N, const double* x, const double*
   i32* st = si + N * D;
   i32* so = st + N;
   i32* sc = so + N;
-  DP_FOR(q, N) {
-    if (!rb[rep[q]]) continue;
+  dp_for_rebuild(plist, ctl, N, [&](const i64 q, const bool valid) {
+    if (!valid || (!plist && !rb[rep[q]])) return;
     const i64 p = perm[q];
     double t[D];
     dp_ldD<D>(x + p * D, t);
@@ -538,7 +559,7 @@ __global__ void k_permute(const i32* perm, i64 N, const double* x, const double*
     st[q] = type[p];
     so[q] = orig[p];
     sc[q] = cell_of[p];
-  }
+  });
 }
 
 // Fixed-point cell-relative coordinate of a position component: the offset
@@ -555,7 +576,7 @@ template <int D>
 __global__ void k_unpermute_store(i64 N, double* x, double* v, double* beta, double* mass, i32* img, i32* type,
                                   i32* orig, i32* cell_of, const char* scratch, double* xref, i32* cellpack,
                                   i32* xq, int rs, double qinv, const i32* rep, const DpRep* reps,
-                                  const DpCtl* ctl, const i32* rb) {
+                                  const DpCtl* ctl, const i32* rb, const int2* plist) {
   if (!ctl->rebuild) return;
   const double* sx = (const double*)scratch;
   const double* sv = sx + N * D;
@@ -565,8 +586,8 @@ __global__ void k_unpermute_store(i64 N, double* x, double* v, double* beta, dou
   const i32* st = si + N * D;
   const i32* so = st + N;
   const i32* sc = so + N;
-  DP_FOR(q, N) {
-    if (!rb[rep[q]]) continue;
+  dp_for_rebuild(plist, ctl, N, [&](const i64 q, const bool valid) {
+    if (!valid || (!plist && !rb[rep[q]])) return;
     const DpRep& R = reps[rep[q]];
     const i32 cid = sc[q];
     int c[D];
@@ -595,7 +616,7 @@ __global__ void k_unpermute_store(i64 N, double* x, double* v, double* beta, dou
       if (D < rs) xq[q * rs + D] = st[q];
       for (int k = D + 1; k < rs; ++k) xq[q * rs + k] = 0;
     }
-  }
+  });
 }
 
 // Mixed records from the current positions (after host uploads).
@@ -1399,6 +1420,124 @@ __global__ void __launch_bounds__(256) k_emit_list(const DpRep* __restrict__ rep
     __syncthreads();
   }
   if (tid == 0) ctl->emit_ntodo = carry;
+}
+
+// Batches: the lists that restrict the rebuild pipeline to the replicas that
+// rebuild in this step (one workgroup; block scans over the replicas): the
+// DP_BLOCK-particle blocks [start, end) of their slot ranges (plist,
+// ctl->pblk_n) and the replicas themselves (rlist, ctl->rb_n).  The other
+// replicas keep their cells, cell starts (their cell counts stay zero) and
+// slots.
+__global__ void __launch_bounds__(256) k_rebuild_list(const DpRep* __restrict__ reps, int R,
+                                                      const i32* __restrict__ rb, int2* plist, i32* rlist,
+                                                      DpCtl* ctl) {
+  __shared__ int sb[256], sr[256];
+  __shared__ int carry_b, carry_r;
+  const int tid = threadIdx.x;
+  if (!ctl->rebuild) return;
+  if (tid == 0) carry_b = carry_r = 0;
+  __syncthreads();
+  for (int base = 0; base < R; base += 256) {
+    const int r = base + tid;
+    const bool on = r < R && rb[r];
+    const int nb = on ? (int)((reps[r].n + DP_BLOCK - 1) / DP_BLOCK) : 0;
+    sb[tid] = nb;
+    sr[tid] = on ? 1 : 0;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      const int vb = tid >= off ? sb[tid - off] : 0;
+      const int vr = tid >= off ? sr[tid - off] : 0;
+      __syncthreads();
+      sb[tid] += vb;
+      sr[tid] += vr;
+      __syncthreads();
+    }
+    if (on) {
+      const int p0 = (int)reps[r].p0, p1 = (int)(reps[r].p0 + reps[r].n);
+      const int first = carry_b + sb[tid] - nb;
+      for (int k = 0; k < nb; ++k) plist[first + k] = make_int2(p0 + k * DP_BLOCK, p1);
+      rlist[carry_r + sr[tid] - 1] = r;
+    }
+    __syncthreads();
+    if (tid == 255) {
+      carry_b += sb[255];
+      carry_r += sr[255];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    ctl->pblk_n = carry_b;
+    ctl->rb_n = carry_r;
+  }
+}
+
+// Batches: the cell-count scan of each rebuilding replica over its own cells
+// [c0, c0 + ncell), starting at its first slot p0 (one workgroup per replica;
+// the counts are cleared as they are read), and the end of its last cell,
+// cstart[c0 + ncell] = p0 + n (the next replica's first cell start, or
+// cstart[C] = N for the last replica).  The replicas that keep their list
+// keep their cell starts.
+__global__ void __launch_bounds__(DP_SCAN_THREADS) k_scan_reps(i32* count, i32* cstart,
+                                                               const DpRep* __restrict__ reps,
+                                                               const i32* __restrict__ rlist, const DpCtl* ctl) {
+  if (!ctl->rebuild) return;
+  __shared__ i32 wsum[DP_SCAN_THREADS / 64];
+  __shared__ i32 btotal;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int w = t >> 6;
+  constexpr int NW = DP_SCAN_THREADS / 64;
+  constexpr int PER = 4;
+  const int nr = ctl->rb_n;
+  for (int j = blockIdx.x; j < nr; j += gridDim.x) {
+    const DpRep& P = reps[rlist[j]];
+    i32* cnt = count + P.c0;
+    i32* cst = cstart + P.c0;
+    const i64 C = P.ncell;
+    i32 carry = (i32)P.p0;
+    for (i64 b0 = 0; b0 < C; b0 += (i64)PER * DP_SCAN_THREADS) {
+      const i64 c0 = b0 + (i64)PER * t;
+      i32 v[PER];
+      i32 sthr = 0;
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        v[k] = (c0 + k < C) ? cnt[c0 + k] : 0;
+        sthr += v[k];
+      }
+      i32 incl = sthr;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const i32 y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+      }
+      if (lane == 63) wsum[w] = incl;
+      __syncthreads();
+      if (w == 0) {
+        const i32 xw = (lane < NW) ? wsum[lane] : 0;
+        i32 xi = xw;
+#pragma unroll
+        for (int off = 1; off < NW; off <<= 1) {
+          const i32 y = __shfl_up(xi, off);
+          if (lane >= off) xi += y;
+        }
+        if (lane < NW) wsum[lane] = xi - xw;
+        if (lane == NW - 1) btotal = xi;
+      }
+      __syncthreads();
+      i32 ex = carry + wsum[w] + (incl - sthr);
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        if (c0 + k < C) {
+          cst[c0 + k] = ex;
+          ex += v[k];
+          cnt[c0 + k] = 0;
+        }
+      }
+      carry += btotal;
+      __syncthreads();  // wsum / btotal are rewritten by the next block
+    }
+    if (t == 0) cst[C] = (i32)(P.p0 + P.n);
+  }
 }
 
 // Emit for the tiles marked by k_emit_tile (halo larger than the LDS record

@@ -354,6 +354,9 @@ struct dp_ctx {
   bool nl_built = false;
   i32* tiletab = nullptr;  // per-tile tables written at every build
   i32* todo = nullptr;     // [ntiles] the emit's tile list (batches)
+  int2* plist = nullptr;   // batches: particle blocks of the rebuilding replicas (k_rebuild_list)
+  i32* rlist = nullptr;    //   and the rebuilding replicas
+  i64 plist_n = 0;
   i64 todo_n = 0;
   int tstride = 0;
   size_t tiletab_bytes = 0;
@@ -864,6 +867,16 @@ int refresh_reps(dp_ctx* ctx) {
     ctx->todo_n = t0;
     release_graph(ctx);  // captured emits carry the list's address
   }
+  {
+    // batches: at most N / DP_BLOCK + R particle blocks of rebuilding replicas
+    const i64 pn = ctx->N / DP_BLOCK + ctx->R + 1;
+    if (pn != ctx->plist_n || !ctx->plist) {
+      CHK(dalloc(ctx, ctx->plist, (size_t)pn));
+      CHK(dalloc(ctx, ctx->rlist, (size_t)std::max(ctx->R, 1)));
+      ctx->plist_n = pn;
+      release_graph(ctx);  // captured rebuilds carry the lists' addresses
+    }
+  }
   // rows padded to a multiple of 8 entries (the pair kernels read 8 bytes at a time)
   const size_t nl_entries = (size_t)ctx->N * (size_t)((ctx->K + 7) & ~7);
   if (nl_entries >= ((size_t)1 << 32))  // the emit indexes the list with 32-bit entry offsets
@@ -1131,9 +1144,19 @@ int launch_rebuild_t(dp_ctx* ctx) {
   hipStream_t st = ctx->st;
   // the cell counts are zero here: cleared at allocation and by the scan
   const i32* rb = ctx->rflag + ctx->R;  // this step's rebuild flag per replica
-  k_bin<D><<<gN, DP_BLOCK, 0, st>>>(ctx->x[cur], ctx->rep, ctx->reps_d, N, ctx->cell_of, ctx->rank, ctx->ccount,
-                                     ctx->ctl_d, rb);
-  {
+  // batches: only the rebuilding replicas' particles (a device-built list of
+  // their particle blocks, a bounded grid) and cells (a scan per replica)
+  const bool listed = ctx->R > 1;
+  const int2* pl = listed ? ctx->plist : nullptr;
+  const int gP = listed ? std::min(gN, 2048) : gN;
+  if (listed)
+    k_rebuild_list<<<1, 256, 0, st>>>(ctx->reps_d, ctx->R, rb, ctx->plist, ctx->rlist, ctx->ctl_d);
+  k_bin<D><<<gP, DP_BLOCK, 0, st>>>(ctx->x[cur], ctx->rep, ctx->reps_d, N, ctx->cell_of, ctx->rank, ctx->ccount,
+                                     ctx->ctl_d, rb, pl);
+  if (listed) {
+    k_scan_reps<<<std::min(ctx->R, 256), DP_SCAN_THREADS, 0, st>>>(ctx->ccount, ctx->cstart, ctx->reps_d, ctx->rlist,
+                                                                   ctx->ctl_d);
+  } else {
     // cell-count scan: one workgroup for few blocks of 16384 cells, three
     // passes over many workgroups otherwise (block sums in the scratch
     // buffer, free until the scatter)
@@ -1151,16 +1174,17 @@ int launch_rebuild_t(dp_ctx* ctx) {
   // the original ids in cell-grouped order go to the head of the scratch
   // buffer, which k_permute overwrites only after the in-cell sort
   i32* keys = (i32*)ctx->scratch;
-  k_scatter<<<gN, DP_BLOCK, 0, st>>>(ctx->cell_of, ctx->rank, ctx->cstart, ctx->orig, N, ctx->tmp, keys, ctx->ctl_d,
-                                     ctx->rep, rb);
-  k_cellsort<<<gN, DP_BLOCK, 0, st>>>(ctx->tmp, keys, ctx->cell_of, ctx->cstart, N, ctx->perm, ctx->ctl_d, ctx->rep,
-                                      rb);
-  k_permute<D><<<gN, DP_BLOCK, 0, st>>>(ctx->perm, N, ctx->x[cur], ctx->v, ctx->beta, ctx->mass, ctx->img,
-                                         ctx->type, ctx->orig, ctx->cell_of, ctx->scratch, ctx->ctl_d, ctx->rep, rb);
-  k_unpermute_store<D><<<gN, DP_BLOCK, 0, st>>>(N, ctx->x[cur], ctx->v, ctx->beta, ctx->mass, ctx->img, ctx->type,
+  k_scatter<<<gP, DP_BLOCK, 0, st>>>(ctx->cell_of, ctx->rank, ctx->cstart, ctx->orig, N, ctx->tmp, keys, ctx->ctl_d,
+                                     ctx->rep, rb, pl);
+  k_cellsort<<<gP, DP_BLOCK, 0, st>>>(ctx->tmp, keys, ctx->cell_of, ctx->cstart, N, ctx->perm, ctx->ctl_d, ctx->rep,
+                                      rb, pl);
+  k_permute<D><<<gP, DP_BLOCK, 0, st>>>(ctx->perm, N, ctx->x[cur], ctx->v, ctx->beta, ctx->mass, ctx->img,
+                                         ctx->type, ctx->orig, ctx->cell_of, ctx->scratch, ctx->ctl_d, ctx->rep, rb,
+                                         pl);
+  k_unpermute_store<D><<<gP, DP_BLOCK, 0, st>>>(N, ctx->x[cur], ctx->v, ctx->beta, ctx->mass, ctx->img, ctx->type,
                                                  ctx->orig, ctx->cell_of, ctx->scratch, ctx->xref, ctx->cellpack,
                                                  ctx->xq[cur], ctx->rs, std::ldexp(1.0, ctx->qbits), ctx->rep,
-                                                 ctx->reps_d, ctx->ctl_d, rb);
+                                                 ctx->reps_d, ctx->ctl_d, rb, pl);
   CHK(launch_emit_t<D>(ctx, 0));
   return DP_OK;
 }
@@ -1713,6 +1737,7 @@ DP_API int dp_destroy(dp_ctx* ctx) {
   dfree(ctx->rep); dfree(ctx->cellpack); dfree(ctx->cell_of); dfree(ctx->rank); dfree(ctx->tmp);
   dfree(ctx->perm); dfree(ctx->nlcnt); dfree(ctx->ccount); dfree(ctx->cstart);
   dfree(ctx->nl); dfree(ctx->tiletab); dfree(ctx->todo); dfree(ctx->scratch); dfree(ctx->stage);
+  dfree(ctx->plist); dfree(ctx->rlist);
   if (ctx->ctl_h) (void)hipHostFree(ctx->ctl_h);
   if (ctx->st) (void)hipStreamDestroy(ctx->st);
   delete ctx;
