@@ -286,6 +286,8 @@ struct DpEv3 {
 struct dp_ctx {
   int dev = 0;
   int ncu = 256;  // compute units of the device
+  int lds_cu = 160 * 1024;  // LDS bytes per compute unit
+  int integ_static_lds = 0;  // static LDS of the integrate kernels
   hipStream_t st = nullptr;
   int D = 3, mode = DP_FP64, R = 1, rs = 4;
   i64 N = 0;
@@ -1026,6 +1028,11 @@ int ensure_module(dp_ctx* ctx) {
   HIPCHK(hipModuleGetFunction(&ctx->f_lf, ctx->mod, "dp_tile_lf"));
   HIPCHK(hipModuleGetFunction(&ctx->f_int_gjf, ctx->mod, "dp_integrate_gjf"));
   HIPCHK(hipModuleGetFunction(&ctx->f_int_lf, ctx->mod, "dp_integrate_lf"));
+  {
+    int sh = 0;
+    if (hipFuncGetAttribute(&sh, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, ctx->f_int_gjf) == hipSuccess && sh >= 0)
+      ctx->integ_static_lds = sh;
+  }
   ctx->mod_key = src;
   return DP_OK;
 }
@@ -1083,11 +1090,28 @@ int launch_pair(dp_ctx* ctx, hipFunction_t f, DpPairArgs& a) {
   return DP_OK;
 }
 
-// Per-particle kernels (integrator with given forces).
+// Per-particle kernels (integrator with given forces): one workgroup per 256
+// particles, at five workgroups per compute unit (the kernels' launch bound),
+// or at four where the last round at five would be under a third full (too
+// few workgroups left to keep HBM busy) and four per unit waste less: rounds
+// x workgroups per unit, the time of a bandwidth-bound launch, is the smaller.
+// At N = 2^20, 4096 workgroups make 4.0 rounds at four per unit against 3.2
+// at five (measured 53.0 -> 49.8 us); at 2^21 the last round at five is 40 %
+// full and four per unit measured 1 % slower.  Four per unit are imposed
+// through the LDS request, more than a fifth of the unit's LDS.
 int launch_flat(dp_ctx* ctx, hipFunction_t f, DpPairArgs& a) {
-  const unsigned grid = (unsigned)((ctx->N + 255) / 256);
+  const i64 nblk = (ctx->N + 255) / 256;
+  const unsigned grid = (unsigned)nblk;
+  const i64 slots = (i64)std::max(ctx->ncu, 1);
+  const i64 t4 = (nblk + 4 * slots - 1) / (4 * slots) * 4, t5 = (nblk + 5 * slots - 1) / (5 * slots) * 5;
+  const i64 last5 = nblk % (5 * slots);  // workgroups in the last round at five per unit (0: a full round)
+  unsigned lds = 0;
+  if (t4 < t5 && last5 != 0 && 3 * last5 < 5 * slots) {
+    const int want = ctx->lds_cu / 5 + 1024 - ctx->integ_static_lds;  // > 1/5 of the unit in all
+    if (want > 0 && ctx->integ_static_lds + want <= ctx->lds_cu / 4) lds = (unsigned)want;
+  }
   void* params[] = {&a};
-  HIPCHK(hipModuleLaunchKernel(f, grid, 1, 1, 256, 1, 1, 0, ctx->st, params, nullptr));
+  HIPCHK(hipModuleLaunchKernel(f, grid, 1, 1, 256, 1, 1, lds, ctx->st, params, nullptr));
   return DP_OK;
 }
 
@@ -1633,9 +1657,12 @@ DP_API int dp_create(dp_ctx** out, int device, int dim, int64_t n_total, int n_r
   if (hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess)
     return bail(ctx->fail(DP_ERR_HIP, "hipStreamCreate failed"));
   {
-    int ncu = 0;
+    int ncu = 0, lds = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
       ctx->ncu = ncu;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) == hipSuccess &&
+        lds > 0)
+      ctx->lds_cu = lds;
   }
   {
     // device address of the allocation-free entry point's scratch, once per device
