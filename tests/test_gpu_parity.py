@@ -443,7 +443,9 @@ def test_inhomogeneous_tiles_beyond_lds_capacity(precision):
     """Half of the box empty: the planner sizes the LDS record areas from the
     mean occupancy, so the dense tiles take the global-memory paths of the
     emit and the pair kernels (the densest still do after the measured halo
-    raised the areas); lists and forces must not change."""
+    raised the areas); lists and forces must not change.  Two lanes per
+    particle keep the tiles of several cells whose halos outgrow the areas
+    (the planner would give this small system 16 lanes and one-cell tiles)."""
     from dompap_amd import Simulation
     rng = np.random.default_rng(31)
     g = np.arange(12)
@@ -461,6 +463,7 @@ def test_inhomogeneous_tiles_beyond_lds_capacity(precision):
     s.particle_types = np.zeros((x.shape[0], 1), dtype=np.int32)
     s.set_pair_potential("4*(r**-12-r**-6)", r_cut=2.5)
     s.set_neighbor_list(skin=0.5, max_number_of_neighbors=256)
+    s.set_kernel_tuning(threads_per_particle=2)
     F = s.get_forces()
     plan = s.get_kernel_plan()
     nl = np.asarray(s.neighbor_list)
