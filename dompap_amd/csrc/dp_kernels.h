@@ -1340,8 +1340,10 @@ __global__ void __launch_bounds__(DP_TILE_THREADS, 4) k_emit_tile(const double* 
     const bool fits = S.fits != 0;
     dp_tile_store<D>(S, P, tiletab + t * (i64)tstride);
     // entry format of the tile: byte offsets (efmt) where the halo fits the
-    // pair kernels' LDS record area (byte offsets then stay below 48 KB);
-    // plain staged-halo indices otherwise
+    // largest LDS record area the pair kernels may be given (cap_pair: 48 KB
+    // of records, so byte offsets stay below 2^16; the area in use may grow
+    // after this emit, when a read-back finds larger halos); plain
+    // staged-halo indices otherwise
     const int efmt_t = ((efmt & 31) != 0 && S.nrec + (efmt >> 8) <= cap_pair) ? efmt : 0;
     if (tid == 0) {
       tiletab[t * (i64)tstride + 6] = fits ? 0 : 1;
