@@ -493,10 +493,15 @@ void size_lds(dp_ctx* ctx) {
   ctx->cap = (int)cap;
   ctx->lds_pair = (size_t)cap * ctx->rec_bytes;
   // emit: 18 bytes per record (an int4 record and a u16 candidate-table
-  // entry), ~7.3 KB of tile tables; with 8-cell tiles four emit workgroups
-  // fit a compute unit instead of three
-  i64 ce = stat;
-  if (want > ce) ce = std::max<i64>(ce, std::min<i64>(want, (i64)((40 * 1024 - 7680) / 18)));
+  // entry), ~7.3 KB of tile tables; four emit workgroups fit a compute unit
+  // up to 1,848 records, so the planned area is held to that too (3-D LJ at
+  // N = 2^20 on 34 cells per side: 1,897 records by the rule, three emit
+  // workgroups per compute unit and 730 us per rebuild; tiles beyond the
+  // area, > 3.4 standard deviations above the mean halo, take the global
+  // path)
+  const i64 ce4 = (i64)((40 * 1024 - 7680) / 18);
+  i64 ce = std::min<i64>(stat, ce4);
+  if (want > ce) ce = std::max<i64>(ce, std::min<i64>(want, ce4));
   ce = std::min<i64>(ce, (i64)(48 * 1024 / 16));
   ce = std::max<i64>(ce, 64);
   if ((int)ce != ctx->cap_emit) release_graph(ctx);
