@@ -494,13 +494,15 @@ void size_lds(dp_ctx* ctx) {
   ctx->lds_pair = (size_t)cap * ctx->rec_bytes;
   // emit: 18 bytes per record (an int4 record and a u16 candidate-table
   // entry), ~7.3 KB of tile tables; four emit workgroups fit a compute unit
-  // up to 1,848 records, so the planned area is held to that too (3-D LJ at
-  // N = 2^20 on 34 cells per side: 1,897 records by the rule, three emit
-  // workgroups per compute unit and 730 us per rebuild; tiles beyond the
-  // area, > 3.4 standard deviations above the mean halo, take the global
-  // path)
+  // up to 1,848 records.  Where that bound still covers the mean halo plus
+  // three standard deviations, the planned area drops its 64-record margin
+  // to stay within it (3-D LJ at N = 2^20 on 34 cells per side: 1,897
+  // records by the rule, three emit workgroups per compute unit and 730 us
+  // per rebuild, against 605 us at 1,848).  Fuller halos keep the rule's
+  // area: with the mean near the bound (4-D, replica batches) most tiles
+  // would take the global path (256 replicas 2.44e9 -> 1.90e9).
   const i64 ce4 = (i64)((40 * 1024 - 7680) / 18);
-  i64 ce = std::min<i64>(stat, ce4);
+  i64 ce = (stat > ce4 && stat - 64 <= ce4) ? ce4 : stat;
   if (want > ce) ce = std::max<i64>(ce, std::min<i64>(want, ce4));
   ce = std::min<i64>(ce, (i64)(48 * 1024 / 16));
   ce = std::max<i64>(ce, 64);
