@@ -294,7 +294,10 @@ def cpu_baseline(wl, x, rebuild_interval, budget_s=20.0):
     box = wl["box"]
     cores = C.threads()
     t0 = time.perf_counter()
-    nl, cnt = C.neighbor_list(x, box, wl["r_cut"] + wl["skin"], wl["K"], "cells")
+    pot = {"4*(r**-12-r**-6)": "lj", "(1-r)**2": "harm2", "r**-12": ("ipl", 12)}[wl["pot"]]
+    sig = np.asarray(wl.get("sigma", [[1.0]]), dtype=np.float64)
+    types = wl.get("types")
+    nl, cnt = C.neighbor_list(x, box, float(sig.max()) * wl["r_cut"] + wl["skin"], wl["K"], "cells")
     t_nl = time.perf_counter() - t0
     v = wl["v"].copy()
     beta = np.zeros_like(v)
@@ -305,7 +308,7 @@ def cpu_baseline(wl, x, rebuild_interval, budget_s=20.0):
     F0 = None
     while steps < 2 or (steps < 10 and t_steps < budget_s / 2):
         t0 = time.perf_counter()
-        F = C.pair(xx, box, nl, "lj", wl["r_cut"])["forces"]
+        F = C.pair(xx, box, nl, pot, wl["r_cut"], sigma=sig, eps=np.ones_like(sig), types=types)["forces"]
         if F0 is None:
             F0 = F
         xi = C.normals(n, dim, 1, steps)
